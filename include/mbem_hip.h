@@ -1,0 +1,162 @@
+/*
+ * mbem_hip.h -- C ABI of libmbem_hip.so, the MI355X (gfx950) implementation of
+ * mbem's dense mollified-BEM hot path.
+ *
+ * Conventions (every entry point):
+ *   - pointers named like arrays are DEVICE pointers unless stated; the small
+ *     coefficient vectors (g[3], c[6]) are HOST pointers, read at the call;
+ *   - sizes and leading dimensions are element counts (int64_t);
+ *   - work is enqueued on `stream` and returns at once (stream-ordered);
+ *   - nothing is allocated except by mbem_invert / mbem_lu_* (rocSOLVER's
+ *     own workspace) -- every buffer is caller-owned;
+ *   - return MBEM_OK, MBEM_EINVAL (bad size, null pointer, non-positive mu),
+ *     or MBEM_EHIP (launch / runtime error).  No exception crosses the ABI.
+ *
+ * Source record: mbem_prepare_sources packs each source triangle into a
+ * structure-of-arrays record of mbem_src_fields() doubles (vertices, the
+ * in-plane frame ex / ey / nhat of tri_kernels.py:136-153, the caller's
+ * normal, eps, a non-degenerate flag), column s of a (fields, ld_src) array.
+ * Every kernel below reads sources through that record.
+ *
+ * Layout: matrices are row-major; a kernel block is written at out[(3f+i) *
+ * ld_out + 3s+j] (tri_kernels.py:40-41), so `out` may point inside a larger
+ * system matrix.  Output updates are out = scale * K + beta * out.
+ */
+#ifndef MBEM_HIP_H
+#define MBEM_HIP_H
+
+#include <hip/hip_runtime_api.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MBEM_OK 0
+#define MBEM_EINVAL 1
+#define MBEM_EHIP 2
+
+/* rows of a block-Jacobi chunk one wavefront applies (mbem_bj_apply tables) */
+#define MBEM_BJ_ROWS_PER_WAVE 4
+
+/* doubles per source record */
+int mbem_src_fields(void);
+
+/* Pack sources: tri_verts (ns,3,3), normals (ns,3), eps (ns) -> src
+ * (mbem_src_fields(), ld_src).  Replaces tri_kernels._precompute_frames
+ * (tri_kernels.py:1052-1065) and the per-source _tri_frame (:136-153). */
+int mbem_prepare_sources(const double *tri_verts, const double *normals, const double *eps,
+                         int64_t ns, double *src, int64_t ld_src, hipStream_t stream);
+
+/* U (force -> displacement) block with coefficients g = u_coeffs(mu, lam)
+ * applied in-loop.  Replaces tri_kernels.u_matrix_direct
+ * (tri_kernels.py:546-565). */
+int mbem_u_block(const double *x_field, int64_t nf, const double *src, int64_t ld_src, int64_t ns,
+                 const double *g, double *out, int64_t ld_out, double scale, double beta,
+                 hipStream_t stream);
+
+/* T (slip -> displacement) block with c = t_coeffs(mu, lam).  Replaces
+ * tri_kernels.t_matrix_direct (tri_kernels.py:568-591).  With rowsum != NULL
+ * also rowsum[9f + 3i + j] += rs_scale * sum_s K[3f+i, 3s+j], the calibration
+ * row sum of backends/dense.py:497-499; store = 0 computes the row sum only
+ * (out unused). */
+int mbem_t_block(const double *x_field, int64_t nf, const double *src, int64_t ld_src, int64_t ns,
+                 const double *c, double *out, int64_t ld_out, double scale, double beta,
+                 double *rowsum, double rs_scale, int store, hipStream_t stream);
+
+/* Geometry-only basis stacks: U (3, 3nf, ld_out) and T (6, 3nf, ld_out), stack
+ * k at out + k * stack_stride.  Replace tri_kernels.u_basis_matrices /
+ * t_basis_matrices (tri_kernels.py:451-492) and their _serial twins
+ * (:502-543). */
+int mbem_u_basis(const double *x_field, int64_t nf, const double *src, int64_t ld_src, int64_t ns,
+                 double *out, int64_t ld_out, int64_t stack_stride, hipStream_t stream);
+int mbem_t_basis(const double *x_field, int64_t nf, const double *src, int64_t ld_src, int64_t ns,
+                 double *out, int64_t ld_out, int64_t stack_stride, hipStream_t stream);
+
+/* Matrix-free displacement, out (nf,3).  Replace tri_kernels.u_disp_contract /
+ * t_disp_contract (tri_kernels.py:976-1049). */
+int mbem_u_disp_contract(const double *x_field, int64_t nf, const double *src, int64_t ld_src,
+                         int64_t ns, const double *density, const double *g, double *out,
+                         double scale, double beta, hipStream_t stream);
+int mbem_t_disp_contract(const double *x_field, int64_t nf, const double *src, int64_t ld_src,
+                         int64_t ns, const double *density, const double *c, double *out,
+                         double scale, double beta, hipStream_t stream);
+
+/* Matrix-free stress, out (nf,3,3).  Replace tri_kernels.kelvin_stress_contract
+ * (tri_kernels.py:1068-1108), dd_stress_contract (:1111-1154, total stress)
+ * and eigenstress_contract (:1217-1260, +C:eps*, full sum). */
+int mbem_kelvin_stress_contract(const double *x_field, int64_t nf, const double *src,
+                                int64_t ld_src, int64_t ns, const double *density, double mu,
+                                double lam, double *out, double scale, double beta,
+                                hipStream_t stream);
+int mbem_dd_stress_contract(const double *x_field, int64_t nf, const double *src, int64_t ld_src,
+                            int64_t ns, const double *density, double mu, double lam, double *out,
+                            double scale, double beta, hipStream_t stream);
+int mbem_eigenstress_contract(const double *x_field, int64_t nf, const double *src,
+                              int64_t ld_src, int64_t ns, const double *density, double mu,
+                              double lam, double *out, double scale, double beta,
+                              hipStream_t stream);
+
+/* Eigenstress on a near list (evaluate.py:298-326 _eigen_near_list): point f
+ * sums source chunks pt_chunk[pt_ptr[f] .. pt_ptr[f+1]), chunk c being the
+ * sources ch_src[ch_ptr[c] .. ch_ptr[c+1]).  All index arrays on device. */
+int mbem_eigenstress_near(const double *x_field, int64_t nf, const double *src, int64_t ld_src,
+                          int64_t ns, const double *density, const int64_t *pt_ptr,
+                          const int64_t *pt_chunk, const int64_t *ch_ptr, const int64_t *ch_src,
+                          double mu, double lam, double *out, double scale, double beta,
+                          hipStream_t stream);
+
+/* Free term on a diagonal block (model/equations.py:566-582 add_block_diagonal
+ * at P0): A[3c+a, 3c+b] += scale * D[c,a,b], A pointing at the block. */
+int mbem_add_block_diag(double *A, int64_t lda, int64_t nc, const double *D, double scale,
+                        hipStream_t stream);
+
+/* y = alpha A x + beta y, A (m, n) row-major with leading dimension lda: the
+ * FGMRES matvec (la/solver.py:178) and the block Gauss-Seidel lower
+ * couplings (la/preconditioner.py:530-541). */
+int mbem_gemv(const double *A, int64_t lda, int64_t m, int64_t n, const double *x, double *y,
+              double alpha, double beta, hipStream_t stream);
+
+/* Arnoldi orthogonalization passes (la/solver.py:179-183): h = V w for the k
+ * rows of V (ldv), and w += alpha V^T h.  mbem_multidot needs a device work
+ * buffer of mbem_multidot_work(k, n) doubles. */
+int64_t mbem_multidot_work(int64_t k, int64_t n);
+int mbem_multidot(const double *V, int64_t ldv, int64_t k, int64_t n, const double *w, double *h,
+                  double *work, hipStream_t stream);
+int mbem_multiaxpy(const double *V, int64_t ldv, int64_t k, int64_t n, const double *h, double *w,
+                   double alpha, hipStream_t stream);
+
+/* y = a x + b y */
+int mbem_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t stream);
+
+/* M (m,m) = A[idx, idx]: a block-Jacobi chunk of the assembled operator
+ * (la/preconditioner.py:445 eval_block(ch, ch)). */
+int mbem_gather_block(const double *A, int64_t lda, const int64_t *idx, int64_t m, double *M,
+                      hipStream_t stream);
+
+/* In-place inverse (LAPACK getrf + getri through rocSOLVER) of a chunk: the
+ * factorization step of the block-Jacobi rung (la/preconditioner.py:448
+ * lu_factor), kept as an explicit inverse so the apply is one GEMV. */
+int mbem_invert(double *M, int64_t m, int32_t *ipiv, int32_t *info, hipStream_t stream);
+
+/* Dense LU of a row-major n x n operator in place and the solve A x = b:
+ * the reference dense solve (backends/dense.py:532-571, la/lu.py:110-239 with
+ * lu="double"). */
+int mbem_lu_factor(double *A, int64_t n, int32_t *ipiv, int32_t *info, hipStream_t stream);
+int mbem_lu_solve(const double *LU, int64_t n, const int32_t *ipiv, double *b, int64_t nrhs,
+                  hipStream_t stream);
+
+/* Block-Jacobi apply (la/preconditioner.py:456-462 + _bj_solve :601-630):
+ * z[idx[c][i]] = sum_j inv_c[i,j] r[idx[c][j]] for every chunk c.  chunk_ptr
+ * (nchunks+1) delimits each chunk's DOF list in idx, inv_ptr its (m_c, m_c)
+ * row-major inverse in inv; wave_table (nwaves, 2) = (chunk, first row) with
+ * MBEM_BJ_ROWS_PER_WAVE rows per wave.  All on device. */
+int mbem_bj_apply(const int64_t *wave_table, int64_t nwaves, const int64_t *chunk_ptr,
+                  const int64_t *idx, const int64_t *inv_ptr, const double *inv, const double *r,
+                  double *z, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MBEM_HIP_H */

@@ -1,0 +1,122 @@
+"""Loader of libmbem_hip.so (the C ABI declared in include/mbem_hip.h).
+
+torch is imported first so that the HIP runtime, rocBLAS and rocSOLVER the
+library links against resolve to the copies torch already loaded (same
+SONAMEs): one runtime per process, so torch-allocated device memory and
+torch streams are valid arguments.  There is no CPU fallback: a missing
+library or a missing GPU raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import torch  # noqa: F401  (must precede the library load, see docstring)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmbem_hip.so")
+
+MBEM_OK, MBEM_EINVAL, MBEM_EHIP = 0, 1, 2
+BJ_ROWS_PER_WAVE = 4
+
+_p = ctypes.c_void_p
+_i64 = ctypes.c_int64
+_d = ctypes.c_double
+_int = ctypes.c_int
+
+# name -> argument types (every entry returns int status unless listed below)
+_SIGS = {
+    "mbem_prepare_sources": [_p, _p, _p, _i64, _p, _i64, _p],
+    "mbem_u_block": [_p, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _p],
+    "mbem_t_block": [_p, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _p, _d, _int, _p],
+    "mbem_u_basis": [_p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p],
+    "mbem_t_basis": [_p, _i64, _p, _i64, _i64, _p, _i64, _i64, _p],
+    "mbem_u_disp_contract": [_p, _i64, _p, _i64, _i64, _p, _p, _p, _d, _d, _p],
+    "mbem_t_disp_contract": [_p, _i64, _p, _i64, _i64, _p, _p, _p, _d, _d, _p],
+    "mbem_kelvin_stress_contract": [_p, _i64, _p, _i64, _i64, _p, _d, _d, _p, _d, _d, _p],
+    "mbem_dd_stress_contract": [_p, _i64, _p, _i64, _i64, _p, _d, _d, _p, _d, _d, _p],
+    "mbem_eigenstress_contract": [_p, _i64, _p, _i64, _i64, _p, _d, _d, _p, _d, _d, _p],
+    "mbem_eigenstress_near": [_p, _i64, _p, _i64, _i64, _p, _p, _p, _p, _p, _d, _d, _p, _d, _d, _p],
+    "mbem_add_block_diag": [_p, _i64, _i64, _p, _d, _p],
+    "mbem_gemv": [_p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
+    "mbem_multidot": [_p, _i64, _i64, _i64, _p, _p, _p, _p],
+    "mbem_multiaxpy": [_p, _i64, _i64, _i64, _p, _p, _d, _p],
+    "mbem_axpby": [_i64, _d, _p, _d, _p, _p],
+    "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
+    "mbem_invert": [_p, _i64, _p, _p, _p],
+    "mbem_lu_factor": [_p, _i64, _p, _p, _p],
+    "mbem_lu_solve": [_p, _i64, _p, _p, _i64, _p],
+    "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
+    "mbem_src_fields": [],
+    "mbem_multidot_work": [_i64, _i64],
+}
+_RESTYPE = {"mbem_multidot_work": _i64}
+
+_lib = None
+
+
+class MbemError(RuntimeError):
+    pass
+
+
+def exported_names() -> list:
+    """Every C entry point this module binds (include/mbem_hip.h)."""
+    return sorted(_SIGS)
+
+
+def load():
+    """The bound library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with "
+                              f"`python -m mbem_amd.build` (hipcc, gfx950)")
+        lib = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            f = getattr(lib, name)
+            f.argtypes = args
+            f.restype = _RESTYPE.get(name, _int)
+        _lib = lib
+    return _lib
+
+
+def require_gpu():
+    if not torch.cuda.is_available():
+        raise MbemError("libmbem_hip needs a ROCm GPU (torch.cuda.is_available() is False); "
+                        "this build has no CPU path")
+    load()
+
+
+def device():
+    require_gpu()
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def stream():
+    """hipStream_t of torch's current stream on the current device."""
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def ptr(t) -> ctypes.c_void_p:
+    if t is None:
+        return ctypes.c_void_p(0)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def check(status: int, what: str) -> None:
+    if status == MBEM_OK:
+        return
+    kind = {MBEM_EINVAL: "invalid argument", MBEM_EHIP: "HIP error"}.get(status, f"status {status}")
+    raise MbemError(f"{what}: {kind}")
+
+
+def call(name: str, *args) -> None:
+    check(getattr(load(), name)(*args), name)
+
+
+def host_doubles(vals):
+    arr = (ctypes.c_double * len(vals))(*[float(v) for v in vals])
+    return arr
+
+
+SRC_FIELDS = 24

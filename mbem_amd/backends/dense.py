@@ -1,0 +1,527 @@
+"""Dense backend on one MI355X: the system matrix assembled in HBM by the HIP
+pair kernels, solved by FGMRES with the block Gauss-Seidel / block-Jacobi
+preconditioner, or by LU (mbem's src/mbem/backends/dense.py).
+
+API as the reference's: ``DenseBackend(mode, jump, deflate, lu).assemble(
+system, eps) -> AssembledDense`` with ``.system .layout .eps .materials .calib
+.A .b .report``, ``.solve() -> {slot name: (n_dofs, 3)}``, ``.release()`` and
+``.rebuild_for_materials(material_map)`` (dense.py:49-86, :358-687).  The
+matrix stays on the device (``A_dev``, ``b_dev``); ``A`` / ``b`` are host
+copies made on first access.  ``solve`` takes the iterative backend's
+keywords (hmat.py:608-616): ``rtol, restart, maxiter, x0, precond_max_dense,
+precond_bj_chunk`` for FGMRES (the default here), or ``method="lu"`` for the
+reference's dense LU.
+
+Assembly (dense.py:480-528): every BlockTerm block is written straight into
+A by one kernel launch with its sign and material applied in-loop; the
+calibrated diagonal's row sums (equations.py:512-537) are fused into the T
+launches, and T blocks needed only for a row sum (a prescribed-displacement
+source patch) are row-summed without being stored.  A row sum is computed
+only where its C_q enters A or b -- rows with a free term (a u-slot) and
+prescribed rows with a nonzero datum; ``calibrate_all_rows=True`` computes
+every C_q as the reference does (the skipped ones never reach A or b).
+"""
+from __future__ import annotations
+
+import time
+import warnings
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from .. import _lib, defaults
+from .._lib import call, ptr, stream
+from ..kernels import KERNEL_T, KERNEL_U
+from ..kernels import basis as kb
+from ..kernels import device as dv
+from ..model.core import BCType
+from ..model.equations import BlockSystem, add_jump_rhs, rhs_terms_at
+
+F64 = torch.float64
+
+
+class DenseBackend:
+    """dense.py:49-86.  ``jump="calibrated"`` annihilates constant fields
+    exactly; ``deflate`` borders out the rigid translations of an all-Neumann
+    model; ``lu`` names the LU flavour of ``solve(method="lu")`` ("double")."""
+
+    def __init__(self, mode: str = "direct", jump: str = "calibrated", deflate: bool = False,
+                 lu: str | None = None, calibrate_all_rows: bool = False):
+        if mode not in ("legacy", "basis", "direct"):
+            raise ValueError(mode)
+        if mode != "direct":
+            raise NotImplementedError(f"dense mode {mode!r}: only 'direct' assembly runs on the "
+                                      f"GPU in this build")
+        if jump not in ("half", "calibrated"):
+            raise ValueError(jump)
+        if lu not in (None, "double"):
+            raise NotImplementedError(f"lu={lu!r}: only FP64 LU ('double') is in this build")
+        self.mode = mode
+        self.jump = jump
+        self.deflate = deflate
+        self.lu = lu
+        self.calibrate_all_rows = calibrate_all_rows
+
+    def assemble(self, system: BlockSystem, eps) -> "AssembledDense":
+        return AssembledDense(system, eps, self.mode, jump=self.jump, deflate=self.deflate,
+                              lu=self.lu, calibrate_all_rows=self.calibrate_all_rows)
+
+
+def require_anchor_or_deflate(system: BlockSystem, jump: str, deflate: bool):
+    """dense.py:101-121."""
+    if jump == "calibrated" and not deflate and not system.model.is_anchored() \
+            and system.layout.has_kind("u"):
+        raise ValueError(
+            "jump='calibrated' on an all-Neumann model (no PRESCRIBED_DISPLACEMENT patch): "
+            "rigid translations are an EXACT null space of the calibrated operator. Pass "
+            "deflate=True (bordered / projected solve) or prescribe displacement on a patch.")
+
+
+def warn_half_jump_eps(system: BlockSystem, eps, jump: str):
+    """dense.py:124-147."""
+    if jump != "half":
+        return
+    worst_name, worst = None, 0.0
+    for r in system.model.regions:
+        for p in list(r.patches) + list(r.faults):
+            try:
+                ratio = float(np.max(kb.resolve_patch_eps(eps, p) / kb.element_sizes(p.mesh)))
+            except ValueError:
+                continue
+            if ratio > worst:
+                worst_name, worst = p.name, ratio
+    if worst > defaults.HALF_JUMP_MAX_EPS_OVER_H:
+        warnings.warn(f"jump='half' with eps/h = {worst:.2f} on patch '{worst_name}' (limit "
+                      f"{defaults.HALF_JUMP_MAX_EPS_OVER_H}): this combination was measured "
+                      f"NON-convergent under h-refinement; use jump='calibrated' or a smaller eps")
+
+
+def warn_collocation_near_fault(system: BlockSystem, eps):
+    """dense.py:150-187."""
+    from ..geometry import distance_to_mesh
+    limit = defaults.COLLOCATION_FAULT_CLEARANCE_EPS
+    n_close, worst, names = 0, np.inf, []
+    for r in system.model.regions:
+        for q in r.patches:
+            x = q.collocation_points()
+            for f in r.faults:
+                try:
+                    eps_f = kb.resolve_patch_eps(eps, f)
+                except ValueError:
+                    continue
+                d, idx = distance_to_mesh(x, f.mesh)
+                ratio = d / eps_f[idx]
+                close = int(np.sum(ratio < limit))
+                if close:
+                    n_close += close
+                    worst = min(worst, float(ratio.min()))
+                    names.append(f"{q.name}/{f.name}")
+    if n_close:
+        warnings.warn(f"{n_close} boundary collocation point(s) lie within {limit:g} fault-eps "
+                      f"of a fault element ({', '.join(names)}; min d/eps = {worst:.2f}): the "
+                      f"row is not corrected")
+
+
+def translation_basis(layout) -> np.ndarray:
+    """Orthonormal rigid-translation basis (n, 3) on the u-slots (dense.py:190-208)."""
+    if not layout.has_kind("u"):
+        return np.zeros((layout.n_unknowns, 0))
+    Z = np.zeros((layout.n_unknowns, 3))
+    for slot in layout.slots:
+        if slot.kind == "u":
+            for k in range(3):
+                Z[slot.offset + k:slot.stop:3, k] = 1.0
+    Z /= np.linalg.norm(Z, axis=0, keepdims=True)
+    return Z
+
+
+@dataclass
+class DenseSolveReport:
+    """dense.py:260-271 (LU solve)."""
+    cond_estimate: float
+    lu: dict | None = None
+    residual: float = float("nan")
+
+    def __str__(self):
+        return f"dense LU: cond ~ {self.cond_estimate:.2e}"
+
+
+@dataclass
+class AssemblyReport:
+    """Where the assembly time went (seconds) and how many pairs ran."""
+    seconds: float
+    t_pairs_stored: int
+    t_pairs_rowsum_only: int
+    u_pairs: int
+    rhs_pairs: int
+
+
+class AssembledDense:
+    def __init__(self, system: BlockSystem, eps, mode: str = "direct", jump: str = "calibrated",
+                 deflate: bool = False, lu=None, calibrate_all_rows: bool = False,
+                 fault_drag=None, _materials=None):
+        if jump not in ("half", "calibrated"):
+            raise ValueError(jump)
+        if fault_drag:
+            raise NotImplementedError("fault drag needs FAULT_TRACTION faults (not in this build)")
+        _lib.require_gpu()
+        require_anchor_or_deflate(system, jump, deflate)
+        warn_half_jump_eps(system, eps, jump)
+        warn_collocation_near_fault(system, eps)
+        self.system = system
+        self.layout = system.layout
+        self.eps = eps
+        self.mode = mode
+        self.jump = jump
+        self.deflate = deflate
+        self.lu = lu
+        self.calibrate_all_rows = calibrate_all_rows
+        self.materials = dict(_materials) if _materials is not None else \
+            {r.name: r.material for r in system.model.regions}
+        self.fault_drag = {}
+        self.report = None
+        self._precond = None
+        self._precond_caps = None
+        self._lu = None
+        self._A_host = None
+        self._b_host = None
+        self.A_dev = None
+        self.b_dev = None
+        self._prepare_geometry()
+        self._build()
+
+    # -- geometry on the device ------------------------------------------
+    def _prepare_geometry(self):
+        from ..selfcheck import ensure_fault_convention
+        if any(r.faults for r in self.system.model.regions):
+            ensure_fault_convention()
+        dev = _lib.device()
+        self.device = dev
+        self._src: dict = {}
+        self._pts: dict = {}
+        self._pts_host: dict = {}
+        self._eps_arr: dict = {}
+        patches = []
+        for r in self.system.model.regions:
+            for p in list(r.patches) + list(r.faults):
+                if id(p) not in self._src:
+                    patches.append(p)
+                    eps_arr = kb.resolve_patch_eps(self.eps, p)
+                    tv, nrm = kb._source_arrays(p.mesh)
+                    self._eps_arr[id(p)] = eps_arr
+                    self._src[id(p)] = dv.Sources(torch.from_numpy(tv).to(dev),
+                                                  torch.from_numpy(nrm).to(dev),
+                                                  torch.from_numpy(eps_arr).to(dev))
+                    x = np.ascontiguousarray(p.collocation_points())
+                    self._pts_host[id(p)] = x
+                    self._pts[id(p)] = torch.from_numpy(x).to(dev)
+        self._patches = patches
+
+    def collocation_points_of(self, patch) -> np.ndarray:
+        return self._pts_host[id(patch)]
+
+    def eps_for(self, patch) -> np.ndarray:
+        return self._eps_arr[id(patch)]
+
+    # -- assembly -----------------------------------------------------------
+    def _coeffs(self, kernel, region):
+        m = self.materials[region.name]
+        return kb.t_coeffs(m.mu, m.lam) if kernel == KERNEL_T else kb.u_coeffs(m.mu, m.lam)
+
+    def _calib_needed(self):
+        """(region, q) pairs whose calibrated C_q enters A or b."""
+        need = set()
+        for t in self.system.terms:
+            if t.diag:
+                need.add((id(t.region), id(t.field_patch)))
+        for rt in self.system.rhs_terms:
+            if rt.add_half_of_vector:
+                need.add((id(rt.region), id(rt.field_patch)))
+        if self.calibrate_all_rows:
+            for r in self.system.model.regions:
+                for q in r.patches:
+                    need.add((id(r), id(q)))
+        return need
+
+    def _build(self):
+        t0 = time.perf_counter()
+        torch.cuda.synchronize()
+        n = self.layout.n_unknowns
+        dev = self.device
+        model = self.system.model
+        A = torch.empty((n, n), dtype=F64, device=dev)
+        lda = A.stride(0)
+        n_ts = n_tr = n_u = n_rhs = 0
+        # calibration buffers (C_q, -sum_p sigma rowsum), zero-initialised
+        self.calib = None
+        calib_dev = {}
+        need = set()
+        if self.jump == "calibrated":
+            need = self._calib_needed()
+            for r in model.regions:
+                for q in r.patches:
+                    if (id(r), id(q)) in need:
+                        calib_dev[(id(r), id(q))] = torch.zeros((q.n_nodes, 3, 3), dtype=F64,
+                                                                device=dev)
+        covered = set()
+        stored_T = set()
+        for t in self.system.terms:
+            r0, c0 = t.row.offset, t.col.offset
+            key = (t.row.name, t.col.name)
+            beta = 1.0 if key in covered else 0.0
+            covered.add(key)
+            x = self._pts[id(t.field_patch)]
+            src = self._src[id(t.source_patch)]
+            out = A[r0:, c0:]
+            c = self._coeffs(t.kernel, t.region)
+            if t.kernel == KERNEL_T:
+                ck = (id(t.region), id(t.field_patch))
+                rs = calib_dev.get(ck) if (ck in calib_dev) else None
+                dv.t_block(x, src, c, out, lda, scale=t.scale, beta=beta, rowsum=rs,
+                           rs_scale=-float(model.orientation(t.region, t.source_patch)))
+                stored_T.add((id(t.region), id(t.field_patch), id(t.source_patch)))
+                n_ts += x.shape[0] * src.ns
+            elif t.kernel == KERNEL_U:
+                dv.u_block(x, src, c, out, lda, scale=t.scale, beta=beta)
+                n_u += x.shape[0] * src.ns
+            else:
+                raise NotImplementedError(f"kernel {t.kernel!r} is not in this build")
+        # zero the blocks no term writes
+        slots = self.layout.slots
+        for rs_ in slots:
+            for cs_ in slots:
+                if (rs_.name, cs_.name) not in covered:
+                    A[rs_.offset:rs_.stop, cs_.offset:cs_.stop].zero_()
+        # row sums of T blocks that are not stored (prescribed-u source patches)
+        if self.jump == "calibrated":
+            for r in model.regions:
+                for q in r.patches:
+                    ck = (id(r), id(q))
+                    if ck not in calib_dev:
+                        continue
+                    for p in r.patches:
+                        if (id(r), id(q), id(p)) in stored_T:
+                            continue
+                        x = self._pts[id(q)]
+                        src = self._src[id(p)]
+                        dv.t_block(x, src, self._coeffs(KERNEL_T, r), None, 0, rowsum=calib_dev[ck],
+                                   rs_scale=-float(model.orientation(r, p)), store=False)
+                        n_tr += x.shape[0] * src.ns
+            self.calib_dev = calib_dev
+        # free terms on the u-block diagonals
+        for t in self.system.terms:
+            if not t.diag:
+                continue
+            blk = A[t.row.offset:, t.col.offset:]
+            nq = t.field_patch.n_nodes
+            if self.jump == "calibrated":
+                D = calib_dev[(id(t.region), id(t.field_patch))]
+            else:
+                D = torch.as_tensor(np.broadcast_to(t.diag * np.eye(3), (nq, 3, 3)).copy(),
+                                    dtype=F64, device=dev)
+            dv.add_block_diag(blk, lda, D)
+        # right-hand side (dense.py:514-524)
+        b = torch.zeros(n, dtype=F64, device=dev)
+        for rt in rhs_terms_at(self.system, self.materials):
+            x = self._pts[id(rt.field_patch)]
+            src = self._src[id(rt.source_patch)]
+            dens = torch.as_tensor(np.asarray(rt.vector, float).reshape(-1, 3), dtype=F64,
+                                   device=dev)
+            out = b[rt.row.offset:rt.row.stop].view(-1, 3)
+            c = self._coeffs(rt.kernel, rt.region)
+            if rt.kernel == KERNEL_T:
+                dv.t_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
+            else:
+                dv.u_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
+            n_rhs += x.shape[0] * src.ns
+        for rv in self.system.rhs_values:
+            b[rv.row.offset:rv.row.stop] += torch.as_tensor(rv.vector, dtype=F64, device=dev)
+        if any(rt.add_half_of_vector for rt in self.system.rhs_terms):
+            calib_host = None if self.jump != "calibrated" else {
+                k: v.cpu().numpy() for k, v in calib_dev.items()}
+            bh = np.zeros(n)
+            add_jump_rhs(self.system, calib_host, bh)
+            b += torch.as_tensor(bh, dtype=F64, device=dev)
+        torch.cuda.synchronize()
+        self.A_dev = A
+        self.b_dev = b
+        self.assembly = AssemblyReport(time.perf_counter() - t0, n_ts, n_tr, n_u, n_rhs)
+        if self.jump == "calibrated":
+            self.calib = _LazyCalib(calib_dev)
+
+    # -- host views ----------------------------------------------------------
+    @property
+    def A(self) -> np.ndarray:
+        if self._A_host is None:
+            self._A_host = self.A_dev.cpu().numpy()
+        return self._A_host
+
+    @property
+    def b(self) -> np.ndarray:
+        if self._b_host is None:
+            self._b_host = self.b_dev.cpu().numpy()
+        return self._b_host
+
+    # -- operators -----------------------------------------------------------
+    def apply_into(self, v, out):
+        call("mbem_gemv", ptr(self.A_dev), self.A_dev.stride(0), self.A_dev.shape[0],
+             self.A_dev.shape[1], ptr(v), ptr(out), 1.0, 0.0, stream())
+        return out
+
+    def matvec(self, v):
+        out = torch.empty_like(v)
+        return self.apply_into(v, out)
+
+    # -- solve -----------------------------------------------------------------
+    def solve(self, method: str = "fgmres", rtol: float = defaults.GMRES_RTOL,
+              restart: int = defaults.GMRES_RESTART, maxiter: int = defaults.GMRES_MAXITER,
+              x0=None, precond_max_dense=None,
+              precond_above_dense: str = defaults.PRECOND_RUNG_ABOVE_DENSE,
+              precond_hodlr_max=None, precond_bj_chunk=None, recycle=None,
+              return_device: bool = False):
+        """{slot name: (n_dofs, 3)}; leaves ``self.report`` (SolveReport for
+        FGMRES, DenseSolveReport for LU)."""
+        if method == "lu":
+            x = self._solve_lu()
+        elif method == "fgmres":
+            x = self._solve_fgmres(rtol, restart, maxiter, x0, precond_max_dense,
+                                   precond_above_dense, precond_bj_chunk, recycle)
+        else:
+            raise ValueError(f"solve method {method!r} (expected 'fgmres' or 'lu')")
+        self.x_dev = x
+        if return_device:
+            return x
+        xh = x.cpu().numpy()
+        return {s.name: xh[s.offset:s.stop].reshape(-1, 3) for s in self.layout.slots}
+
+    def preconditioner(self, max_dense=None, bj_chunk=None,
+                       above_dense=defaults.PRECOND_RUNG_ABOVE_DENSE):
+        from ..la.preconditioner import BlockGaussSeidel
+        caps = (max_dense, above_dense, bj_chunk)
+        if self._precond is not None and self._precond_caps != caps:
+            self._precond = None
+        if self._precond is None:
+            self._precond = BlockGaussSeidel(self, max_dense=max_dense, bj_chunk=bj_chunk,
+                                             above_dense=above_dense)
+            self._precond_caps = caps
+        return self._precond
+
+    def _solve_fgmres(self, rtol, restart, maxiter, x0, max_dense, above_dense, bj_chunk,
+                      recycle):
+        from ..la.solver import fgmres
+        M = self.preconditioner(max_dense, bj_chunk, above_dense)
+        if x0 is not None and not isinstance(x0, torch.Tensor):
+            x0 = torch.as_tensor(np.asarray(x0, float), dtype=F64, device=self.device)
+        if self.deflate:
+            Z = torch.as_tensor(translation_basis(self.layout), dtype=F64, device=self.device)
+
+            def proj(v):
+                return v - Z @ (Z.T @ v)
+
+            class _P:
+                def __init__(s, f):
+                    s.f = f
+
+                def apply_into(s, v, out):
+                    out.copy_(proj(s.f(proj(v))))
+                    return out
+
+            x, report = fgmres(_P(self.matvec), proj(self.b_dev), M=_P(M), x0=x0, rtol=rtol,
+                               restart=restart, maxiter=maxiter, recycle=recycle)
+            x = proj(x)
+        else:
+            x, report = fgmres(self, self.b_dev, M=M, x0=x0, rtol=rtol, restart=restart,
+                               maxiter=maxiter, recycle=recycle)
+        report.precond_summary = M.summary()
+        self.report = report
+        return x
+
+    def _solve_lu(self):
+        """LU with partial pivoting on the device (dense.py:532-571, lu.py:110-161)."""
+        n = self.A_dev.shape[0]
+        dev = self.device
+        if self._lu is None:
+            Z = translation_basis(self.layout) if self.deflate else None
+            if Z is not None and Z.shape[1] == 0:
+                Z = None
+            if Z is not None:
+                K = torch.zeros((n + 3, n + 3), dtype=F64, device=dev)
+                K[:n, :n] = self.A_dev
+                Zd = torch.as_tensor(Z, dtype=F64, device=dev)
+                K[:n, n:] = Zd
+                K[n:, :n] = Zd.T
+                LU = K
+            else:
+                LU = self.A_dev.clone()
+            m = LU.shape[0]
+            ipiv = torch.empty(m, dtype=torch.int32, device=dev)
+            info = torch.zeros(1, dtype=torch.int32, device=dev)
+            t0 = time.perf_counter()
+            call("mbem_lu_factor", ptr(LU), m, ptr(ipiv), ptr(info), stream())
+            if int(info.item()) != 0:
+                raise np.linalg.LinAlgError(f"dense LU: singular (getrf info {int(info.item())})")
+            self._lu = (LU, ipiv, m, time.perf_counter() - t0)
+        LU, ipiv, m, fs = self._lu
+        rhs = torch.zeros(m, dtype=F64, device=dev)
+        rhs[:n] = self.b_dev
+        call("mbem_lu_solve", ptr(LU), m, ptr(ipiv), ptr(rhs), 1, stream())
+        x = rhs[:n].contiguous()
+        r = self.b_dev - self.matvec(x)
+        res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(self.b_dev))
+        self.report = DenseSolveReport(cond_estimate=float("nan"),
+                                       lu={"how": "double", "factor_s": fs}, residual=res)
+        return x
+
+    def release(self):
+        """Drop the factors and the preconditioner (dense.py:573-576)."""
+        self._lu = None
+        self._precond = None
+
+    def rebuild_for_materials(self, material_map: dict, fault_drag=None) -> "AssembledDense":
+        """A new assembly of the same system with some regions' materials
+        replaced (dense.py:650-687), re-integrated on the device."""
+        mats = dict(self.materials)
+        for name, mat in material_map.items():
+            if name not in mats:
+                raise KeyError(f"unknown region '{name}'")
+            mats[name] = mat
+        return AssembledDense(self.system, self.eps, self.mode, jump=self.jump,
+                              deflate=self.deflate, lu=self.lu,
+                              calibrate_all_rows=self.calibrate_all_rows, _materials=mats)
+
+
+class _LazyCalib(dict):
+    """``calib`` as the reference exposes it ({(id(region), id(q)): (n,3,3)}),
+    copied to the host on first access."""
+
+    def __init__(self, dev_map):
+        super().__init__()
+        self._dev = dev_map
+
+    def _fill(self):
+        if not super().__len__() and self._dev:
+            for k, v in self._dev.items():
+                super().__setitem__(k, v.cpu().numpy())
+
+    def __getitem__(self, k):
+        self._fill()
+        return super().__getitem__(k)
+
+    def __contains__(self, k):
+        return k in self._dev
+
+    def __len__(self):
+        return len(self._dev)
+
+    def keys(self):
+        self._fill()
+        return super().keys()
+
+    def items(self):
+        self._fill()
+        return super().items()
+
+    def values(self):
+        self._fill()
+        return super().values()

@@ -1,0 +1,307 @@
+// FP64 linear algebra of the dense solve on gfx950:
+//  * y = alpha A x + beta y, A row-major (the FGMRES matvec and the
+//    Gauss-Seidel lower couplings): HBM-bound, one wavefront per R rows,
+//    16-byte non-temporal loads of A, x reused from L2 across the R rows;
+//  * split-K multi-dot h = V w and multi-axpy w += alpha V^T h, the two passes
+//    of classical Gram-Schmidt (run twice) in the Arnoldi step;
+//  * block-Jacobi: chunk gather + LAPACK getrf/getri (rocSOLVER) at build,
+//    and one launch applying every chunk's explicit inverse with the
+//    gather/scatter of its DOF list fused in.
+#include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
+#include <stdint.h>
+
+#include "../../include/mbem_hip.h"
+
+namespace {
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+inline int launch_status() { return hipGetLastError() == hipSuccess ? MBEM_OK : MBEM_EHIP; }
+
+// y[r] = alpha * sum_j A[r, j] x[j] + beta * y[r]; VEC: 16-byte aligned rows
+template <int R, bool VEC>
+__global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, long lda, long m,
+                                              long n, const double* __restrict__ x,
+                                              double* __restrict__ y, double alpha, double beta)
+{
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long r0 = wave * R;
+    if (r0 >= m) return;
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) acc[r] = 0.0;
+    const double* rows[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) rows[r] = A + ((r0 + r < m) ? r0 + r : m - 1) * lda;
+    if (VEC) {
+        const long n2 = n & ~1L;
+        long j = 2 * lane;
+        for (; j + 128 < n2; j += 256) {
+            dbl2 xa = *reinterpret_cast<const dbl2*>(x + j);
+            dbl2 xb = *reinterpret_cast<const dbl2*>(x + j + 128);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                dbl2 a = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(rows[r] + j));
+                dbl2 b = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(rows[r] + j + 128));
+                acc[r] += a.x * xa.x + a.y * xa.y + b.x * xb.x + b.y * xb.y;
+            }
+        }
+        for (; j < n2; j += 128) {
+            dbl2 xa = *reinterpret_cast<const dbl2*>(x + j);
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                dbl2 a = __builtin_nontemporal_load(reinterpret_cast<const dbl2*>(rows[r] + j));
+                acc[r] += a.x * xa.x + a.y * xa.y;
+            }
+        }
+        if ((n & 1) && lane == 0) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] += rows[r][n - 1] * x[n - 1];
+        }
+    } else {
+        for (long j = lane; j < n; j += 64) {
+            double xv = x[j];
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r] += rows[r][j] * xv;
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        double t = wave_sum(acc[r]);
+        if (lane == 0 && r0 + r < m) {
+            double* p = y + r0 + r;
+            *p = (beta == 0.0) ? alpha * t : alpha * t + beta * (*p);
+        }
+    }
+}
+
+// split-K partial dots: part[chunk * k + row] = sum_{j in chunk} V[row, j] w[j]
+__global__ __launch_bounds__(256) void k_multidot_part(const double* __restrict__ V, long ldv,
+                                                       long k, long n, const double* __restrict__ w,
+                                                       long chunk, double* __restrict__ part)
+{
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long c = blockIdx.y;
+    if (row >= k) return;
+    const long j0 = c * chunk, j1 = (j0 + chunk < n) ? j0 + chunk : n;
+    double acc = 0.0;
+    const double* v = V + row * ldv;
+    for (long j = j0 + lane; j < j1; j += 64) acc += v[j] * w[j];
+    acc = wave_sum(acc);
+    if (lane == 0) part[c * k + row] = acc;
+}
+
+__global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__ part, long k,
+                                                      long nparts, double* __restrict__ h)
+{
+    long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= k) return;
+    double s = 0.0;
+    for (long c = 0; c < nparts; ++c) s += part[c * k + row];
+    h[row] = s;
+}
+
+// w[j] += alpha * sum_row V[row, j] h[row]
+__global__ __launch_bounds__(256) void k_multiaxpy(const double* __restrict__ V, long ldv, long k,
+                                                   long n, const double* __restrict__ h,
+                                                   double* __restrict__ w, double alpha)
+{
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    double s = 0.0;
+    for (long row = 0; row < k; ++row) s += V[row * ldv + j] * h[row];
+    w[j] += alpha * s;
+}
+
+__global__ __launch_bounds__(256) void k_axpby(long n, double a, const double* __restrict__ x,
+                                               double b, double* __restrict__ y)
+{
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    y[j] = (b == 0.0) ? a * x[j] : a * x[j] + b * y[j];
+}
+
+// chunk gather: M[i, j] = A[idx[i], idx[j]]
+__global__ __launch_bounds__(256) void k_gather_block(const double* __restrict__ A, long lda,
+                                                      const int64_t* __restrict__ idx, long m,
+                                                      double* __restrict__ M)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m * m) return;
+    long i = t / m, j = t % m;
+    M[t] = A[idx[i] * lda + idx[j]];
+}
+
+// block-Jacobi apply: wave w of the launch takes rows [row0, row0 + R) of
+// chunk wt[w] (table rows: chunk id, first row); z[idx[i]] = sum_j
+// inv[i, j] r[idx[j]] for each row i of that chunk.
+template <int R>
+__global__ __launch_bounds__(256) void k_bj_apply(const int64_t* __restrict__ wtab, long nwaves,
+                                                  const int64_t* __restrict__ cptr,
+                                                  const int64_t* __restrict__ idx,
+                                                  const int64_t* __restrict__ iptr,
+                                                  const double* __restrict__ inv,
+                                                  const double* __restrict__ r,
+                                                  double* __restrict__ z)
+{
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= nwaves) return;
+    const long c = wtab[2 * wave], row0 = wtab[2 * wave + 1];
+    const long i0 = cptr[c], m = cptr[c + 1] - i0;
+    const int64_t* id = idx + i0;
+    const double* B = inv + iptr[c];
+    double acc[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc[q] = 0.0;
+    for (long j = lane; j < m; j += 64) {
+        double rv = r[id[j]];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            long i = row0 + q;
+            if (i < m) acc[q] += __builtin_nontemporal_load(B + i * m + j) * rv;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        double t = wave_sum(acc[q]);
+        long i = row0 + q;
+        if (lane == 0 && i < m) z[id[i]] = t;
+    }
+}
+
+rocblas_handle g_handle = nullptr;
+
+}  // namespace
+
+extern "C" {
+
+int mbem_gemv(const double* A, int64_t lda, int64_t m, int64_t n, const double* x, double* y,
+              double alpha, double beta, hipStream_t stream)
+{
+    if (m < 0 || n < 0 || (m > 0 && (!A || !y || lda < n)) || (n > 0 && !x)) return MBEM_EINVAL;
+    if (m == 0) return MBEM_OK;
+    constexpr int R = 4;
+    long waves = (m + R - 1) / R;
+    dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    bool vec = ((reinterpret_cast<uintptr_t>(A) & 15) == 0) && ((lda & 1) == 0)
+               && ((reinterpret_cast<uintptr_t>(x) & 15) == 0);
+    if (vec)
+        hipLaunchKernelGGL((k_gemv<R, true>), grid, block, 0, stream, A, lda, m, n, x, y, alpha, beta);
+    else
+        hipLaunchKernelGGL((k_gemv<R, false>), grid, block, 0, stream, A, lda, m, n, x, y, alpha, beta);
+    return launch_status();
+}
+
+int64_t mbem_multidot_work(int64_t k, int64_t n)
+{
+    const int64_t chunk = 4096;
+    int64_t parts = (n + chunk - 1) / chunk;
+    return parts * (k > 0 ? k : 1);
+}
+
+int mbem_multidot(const double* V, int64_t ldv, int64_t k, int64_t n, const double* w, double* h,
+                  double* work, hipStream_t stream)
+{
+    if (k < 0 || n < 0 || (k > 0 && (!V || !h || !work || ldv < n)) || (n > 0 && !w)) return MBEM_EINVAL;
+    if (k == 0) return MBEM_OK;
+    const int64_t chunk = 4096;
+    int64_t parts = (n + chunk - 1) / chunk;
+    if (parts == 0) parts = 1;
+    dim3 grid((unsigned)((k + 3) / 4), (unsigned)parts), block(256);
+    hipLaunchKernelGGL(k_multidot_part, grid, block, 0, stream, V, ldv, k, n, w, chunk, work);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, stream, work, k, parts, h);
+    return launch_status();
+}
+
+int mbem_multiaxpy(const double* V, int64_t ldv, int64_t k, int64_t n, const double* h, double* w,
+                   double alpha, hipStream_t stream)
+{
+    if (k < 0 || n < 0 || (k > 0 && n > 0 && (!V || !h || !w || ldv < n))) return MBEM_EINVAL;
+    if (k == 0 || n == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_multiaxpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, V, ldv, k, n, h, w, alpha);
+    return launch_status();
+}
+
+int mbem_axpby(int64_t n, double a, const double* x, double b, double* y, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!x || !y))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, a, x, b, y);
+    return launch_status();
+}
+
+int mbem_gather_block(const double* A, int64_t lda, const int64_t* idx, int64_t m, double* M,
+                      hipStream_t stream)
+{
+    if (m < 0 || (m > 0 && (!A || !idx || !M))) return MBEM_EINVAL;
+    if (m == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_gather_block, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, stream, A, lda, idx, m, M);
+    return launch_status();
+}
+
+// In-place inverse of an m x m matrix (row-major or column-major alike: the
+// inverse of the transpose is the transpose of the inverse).  ipiv: m int32
+// on the device, info: 1 int32 on the device.
+int mbem_invert(double* M, int64_t m, int32_t* ipiv, int32_t* info, hipStream_t stream)
+{
+    if (m < 0 || (m > 0 && (!M || !ipiv || !info)) || m > INT32_MAX) return MBEM_EINVAL;
+    if (m == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    rocblas_int mm = (rocblas_int)m;
+    if (rocsolver_dgetrf(g_handle, mm, mm, M, mm, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
+    if (rocsolver_dgetri(g_handle, mm, M, mm, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
+    return launch_status();
+}
+
+// LU factorization of a row-major n x n matrix in place (LAPACK getrf of its
+// column-major view, i.e. of A^T), and the matching solve A x = b (getrs with
+// transpose on that factor).  ipiv: n int32, info: 1 int32, both on device.
+int mbem_lu_factor(double* A, int64_t n, int32_t* ipiv, int32_t* info, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!A || !ipiv || !info)) || n > INT32_MAX) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    if (rocsolver_dgetrf(g_handle, (rocblas_int)n, (rocblas_int)n, A, (rocblas_int)n, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
+    return launch_status();
+}
+
+int mbem_lu_solve(const double* LU, int64_t n, const int32_t* ipiv, double* b, int64_t nrhs,
+                  hipStream_t stream)
+{
+    if (n < 0 || nrhs < 0 || (n > 0 && (!LU || !ipiv || !b))) return MBEM_EINVAL;
+    if (n == 0 || nrhs == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    if (rocsolver_dgetrs(g_handle, rocblas_operation_transpose, (rocblas_int)n, (rocblas_int)nrhs,
+                         const_cast<double*>(LU), (rocblas_int)n, const_cast<int32_t*>(ipiv), b,
+                         (rocblas_int)n) != rocblas_status_success)
+        return MBEM_EHIP;
+    return launch_status();
+}
+
+int mbem_bj_apply(const int64_t* wave_table, int64_t nwaves, const int64_t* chunk_ptr,
+                  const int64_t* idx, const int64_t* inv_ptr, const double* inv, const double* r,
+                  double* z, hipStream_t stream)
+{
+    if (nwaves < 0 || (nwaves > 0 && (!wave_table || !chunk_ptr || !idx || !inv_ptr || !inv || !r || !z))) return MBEM_EINVAL;
+    if (nwaves == 0) return MBEM_OK;
+    dim3 grid((unsigned)((nwaves + 3) / 4)), block(256);
+    hipLaunchKernelGGL((k_bj_apply<MBEM_BJ_ROWS_PER_WAVE>), grid, block, 0, stream, wave_table, nwaves, chunk_ptr, idx, inv_ptr, inv, r, z);
+    return launch_status();
+}
+
+}  // extern "C"

@@ -1,0 +1,249 @@
+"""Interior displacement and stress readout on the GPU (mbem's
+src/mbem/evaluate.py:194-253, :256-407, :598-695).
+
+Representation for x inside region R (evaluate.py:1-31):
+
+    u(x) = sum_p sigma G t_p - sum_p sigma H u_p - sum_f sigma_f H slip_f
+    sigma(x) = sum_p sigma SG t_p - sum_p sigma SH u_p - sum_f sigma_f SH slip_f
+
+with every double layer's eigenstress +sigma C:eps* added back when
+``subtract_anelastic`` (the exact finite-triangle form, summed over the
+near list of EIGEN_NEAR_RADIUS_EPS * (eps + h) per element, or over every
+element with ``near_list=False``).  A point ON a boundary patch reads HALF
+the interior limit (evaluate.py:629-636).  All accumulation happens in one
+device buffer; the host sees the result once.
+"""
+from __future__ import annotations
+
+import warnings
+
+import numpy as np
+import torch
+
+from . import _lib, defaults
+from .kernels import basis as kb
+from .kernels import device as dv
+from .model.core import BCType, RegionModel
+
+F64 = torch.float64
+
+
+def _region(model: RegionModel, region):
+    """The MODEL's region of that name (evaluate.py:74-88)."""
+    name = region if isinstance(region, str) else region.name
+    for r in model.regions:
+        if r.name == name:
+            return r
+    raise KeyError(f"no region named {name!r} (regions: {[r.name for r in model.regions]})")
+
+
+def _density(model: RegionModel, patch, kind: str, solution: dict, region=None) -> np.ndarray:
+    """evaluate.py:91-119: the prescribed value where the BC fixes it, else
+    the solved slot; "slip" is a fault's prescribed Burgers vector."""
+    if kind == "slip":
+        return patch.density()
+    fixed = {"u": BCType.PRESCRIBED_DISPLACEMENT, "t": BCType.FREE_TRACTION}
+    if kind not in fixed:
+        raise ValueError(f"unknown density kind {kind!r}")
+    if patch.bc is fixed[kind]:
+        return model.value_of(patch)
+    return np.asarray(solution[f"{kind}:{patch.name}"], float)
+
+
+def _warn_near_boundary(points, region):
+    """evaluate.py:135-167."""
+    from .geometry import distance_to_mesh
+    ratio_max = defaults.NEAR_BOUNDARY_H_RATIO
+    points = np.asarray(points, float).reshape(-1, 3)
+    close = np.zeros(points.shape[0], bool)
+    worst, names = np.inf, []
+    for p in region.patches:
+        d, idx = distance_to_mesh(points, p.mesh)
+        ratio = d / kb.element_sizes(p.mesh)[idx]
+        if ratio.size and ratio.min() < ratio_max:
+            close |= ratio < ratio_max
+            worst = min(worst, float(ratio.min()))
+            names.append(p.name)
+    if close.any():
+        warnings.warn(f"{int(close.sum())} of {close.size} observation point(s) lie within "
+                      f"{ratio_max}*local-h of boundary patch(es) {', '.join(names)} (min d/h = "
+                      f"{worst:.2f}): the piecewise-constant boundary density limits the volume "
+                      f"representation there")
+
+
+class _Ctx:
+    """Device copies shared by one readout call."""
+
+    def __init__(self, points, eps):
+        self.dev = _lib.device()
+        self.x = torch.from_numpy(np.ascontiguousarray(points, dtype=np.float64)).to(self.dev)
+        self.eps = eps
+        self._src = {}
+
+    def sources(self, patch):
+        k = id(patch)
+        if k not in self._src:
+            tv, nrm = kb._source_arrays(patch.mesh)
+            e = kb.resolve_patch_eps(self.eps, patch)
+            self._src[k] = (dv.Sources(torch.from_numpy(tv).to(self.dev),
+                                       torch.from_numpy(nrm).to(self.dev),
+                                       torch.from_numpy(e).to(self.dev)), tv, nrm, e)
+        return self._src[k]
+
+    def dens(self, d):
+        return torch.from_numpy(np.ascontiguousarray(np.asarray(d, float).reshape(-1, 3))).to(self.dev)
+
+
+def evaluate_displacement(model: RegionModel, region, solution: dict, points, eps,
+                          warn_near: bool = True, near=None) -> np.ndarray:
+    """Displacement (N,3) at interior ``points`` of ``region`` (evaluate.py:194-253)."""
+    if near is not None:
+        raise NotImplementedError("near-face recovery (near=) is not in this build")
+    region = _region(model, region)
+    if region.faults:
+        from .selfcheck import ensure_fault_convention
+        ensure_fault_convention()
+    points = np.ascontiguousarray(np.asarray(points, dtype=float).reshape(-1, 3))
+    if warn_near:
+        _warn_near_boundary(points, region)
+    ctx = _Ctx(points, eps)
+    out = torch.zeros((points.shape[0], 3), dtype=F64, device=ctx.dev)
+    mat = region.material
+    g = kb.u_coeffs(mat.mu, mat.lam)
+    c = kb.t_coeffs(mat.mu, mat.lam)
+    if points.shape[0]:
+        for p in region.patches:
+            sigma = float(model.orientation(region, p))
+            src = ctx.sources(p)[0]
+            u_p = _density(model, p, "u", solution, region)
+            if np.any(u_p):
+                dv.t_disp(ctx.x, src, ctx.dens(u_p), c, out, scale=-sigma, beta=1.0)
+            t_p = _density(model, p, "t", solution)
+            if np.any(t_p):
+                dv.u_disp(ctx.x, src, ctx.dens(t_p), g, out, scale=sigma, beta=1.0)
+        for f in region.faults:
+            sigma = float(model.orientation(region, f))
+            slip = _density(model, f, "slip", solution)
+            if np.any(slip):
+                dv.t_disp(ctx.x, ctx.sources(f)[0], ctx.dens(slip), c, out, scale=-sigma, beta=1.0)
+    torch.cuda.current_stream().synchronize()
+    return out.cpu().numpy()
+
+
+def _source_chunks(cen, radius):
+    """Spatially compact source chunks for the near list (evaluate.py:256-275):
+    median bisection along the longest box axis until a chunk's box diagonal
+    is within the smallest near radius in it; stack order."""
+    chunks, stack = [], [np.arange(cen.shape[0])]
+    while stack:
+        idx = stack.pop()
+        c = cen[idx]
+        span = c.max(axis=0) - c.min(axis=0)
+        if idx.size <= 1 or np.linalg.norm(span) <= radius[idx].min():
+            chunks.append(idx)
+            continue
+        order = idx[np.argsort(c[:, int(span.argmax())], kind="stable")]
+        stack.append(order[: idx.size // 2])
+        stack.append(order[idx.size // 2:])
+    return chunks
+
+
+def near_list(points, cen, radius):
+    """CSR near list (evaluate.py:278-295): for every chunk S the points P
+    within the one ball covering its members' balls.  Returns (chunks,
+    pt_ptr, pt_chunk): point p's chunks, in chunk order, are
+    pt_chunk[pt_ptr[p]:pt_ptr[p+1]]."""
+    from scipy.spatial import cKDTree
+    n = points.shape[0]
+    chunks = []
+    if n == 0 or cen.shape[0] == 0:
+        return chunks, np.zeros(n + 1, np.int64), np.zeros(0, np.int64)
+    tree = cKDTree(points)
+    pts_l, ch_l = [], []
+    for S in _source_chunks(cen, radius):
+        c0 = cen[S].mean(axis=0)
+        reach = float(np.max(np.linalg.norm(cen[S] - c0, axis=1) + radius[S]))
+        P = np.asarray(tree.query_ball_point(c0, reach), dtype=np.int64)
+        if P.size:
+            pts_l.append(P)
+            ch_l.append(np.full(P.size, len(chunks), np.int64))
+            chunks.append(S)
+    if not chunks:
+        return chunks, np.zeros(n + 1, np.int64), np.zeros(0, np.int64)
+    P = np.concatenate(pts_l)
+    C = np.concatenate(ch_l)
+    order = np.lexsort((C, P))
+    counts = np.bincount(P, minlength=n)
+    pt_ptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+    return chunks, pt_ptr, C[order]
+
+
+def _eigen_near(ctx, points, patch, density, sigma, mu, lam, out):
+    """+sigma * eigenstress over the near pairs (evaluate.py:298-326)."""
+    src, tv, nrm, eps_arr = ctx.sources(patch)
+    dens = np.asarray(density, float).reshape(-1, 3)
+    live = np.flatnonzero((eps_arr > 0.0) & np.any(dens != 0.0, axis=1))
+    if live.size == 0 or points.shape[0] == 0:
+        return
+    cen = tv.mean(axis=1)
+    radius = defaults.EIGEN_NEAR_RADIUS_EPS * (eps_arr + kb.element_sizes(patch.mesh))
+    chunks, pt_ptr, pt_chunk = near_list(points, cen[live], radius[live])
+    if not chunks:
+        return
+    ch_src = np.concatenate([live[S] for S in chunks]).astype(np.int64)
+    ch_ptr = np.concatenate([[0], np.cumsum([S.size for S in chunks])]).astype(np.int64)
+    dv.eigen_near(ctx.x, src, ctx.dens(dens), pt_ptr, pt_chunk, ch_ptr, ch_src, mu, lam, out,
+                  scale=sigma, beta=1.0)
+
+
+def evaluate_stress(model: RegionModel, region, solution: dict, points, eps,
+                    subtract_anelastic: bool = True, warn_near: bool = True,
+                    parts: bool = False, near=None, near_list: bool = True):
+    """Stress (N,3,3) at ``points`` of ``region`` (evaluate.py:598-695);
+    ``parts`` also returns the eigenstress removed."""
+    if near is not None:
+        raise NotImplementedError("near-face recovery (near=) is not in this build")
+    region = _region(model, region)
+    if region.faults:
+        from .selfcheck import ensure_fault_convention
+        ensure_fault_convention("stress")
+    points = np.ascontiguousarray(np.asarray(points, dtype=float).reshape(-1, 3))
+    if warn_near:
+        _warn_near_boundary(points, region)
+    ctx = _Ctx(points, eps)
+    N = points.shape[0]
+    sig = torch.zeros((N, 3, 3), dtype=F64, device=ctx.dev)
+    eig = torch.zeros((N, 3, 3), dtype=F64, device=ctx.dev) if parts else None
+    mu, lam = region.material.mu, region.material.lam
+
+    def double_layer(patch, jump, sigma):
+        src = ctx.sources(patch)[0]
+        d = ctx.dens(jump)
+        dv.stress("dd", ctx.x, src, d, mu, lam, sig, scale=-sigma, beta=1.0)
+        if subtract_anelastic or parts:
+            target = eig if parts else sig
+            if near_list:
+                _eigen_near(ctx, points, patch, jump, sigma, mu, lam, target)
+            else:
+                dv.stress("eigen", ctx.x, src, d, mu, lam, target, scale=sigma, beta=1.0)
+
+    if N:
+        for p in region.patches:
+            sigma = float(model.orientation(region, p))
+            u_p = _density(model, p, "u", solution, region)
+            if np.any(u_p):
+                double_layer(p, u_p, sigma)
+            t_p = _density(model, p, "t", solution)
+            if np.any(t_p):
+                dv.stress("force", ctx.x, ctx.sources(p)[0], ctx.dens(t_p), mu, lam, sig,
+                          scale=sigma, beta=1.0)
+        for f in region.faults:
+            sigma = float(model.orientation(region, f))
+            slip = _density(model, f, "slip", solution)
+            if np.any(slip):
+                double_layer(f, slip, sigma)
+    if parts and subtract_anelastic:
+        sig += eig
+    torch.cuda.current_stream().synchronize()
+    s = sig.cpu().numpy()
+    return (s, eig.cpu().numpy()) if parts else s

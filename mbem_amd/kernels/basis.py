@@ -1,0 +1,183 @@
+"""Material coefficients, eps resolution and the assembly entry points
+(mbem's src/mbem/kernels/basis.py, order 0).
+
+The influence matrices decompose as ``M(material) = sum_k c_k(mu, lam) B_k``
+with geometry-only basis blocks (3 for U, 6 for T).  Coefficients come from
+(mu, lam) directly, never through 1/(1-2nu) (basis.py:14-19).  eps is an
+(N_src,) per-element array everywhere; a scalar is promoted.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .. import defaults
+
+
+def _mu_lam(material=None, mu=None, lam=None):
+    if material is not None:
+        return material.mu, material.lam
+    if mu is None or lam is None:
+        raise ValueError("provide either material or (mu, lam)")
+    return mu, lam
+
+
+def u_coeffs(mu, lam) -> np.ndarray:
+    """(3,) coefficients of the U basis [I1 d, eps^2 I3 d, T2[3]] (basis.py:46-55)."""
+    nu = lam / (2.0 * (lam + mu))
+    one_minus_nu = (lam + 2.0 * mu) / (2.0 * (lam + mu))
+    C1 = 1.0 / (16.0 * np.pi * mu * one_minus_nu)
+    c34 = 3.0 - 4.0 * nu
+    return np.array([c34 * C1, 1.0 / (8.0 * np.pi * mu), C1])
+
+
+def t_coeffs(mu, lam) -> np.ndarray:
+    """(6,) coefficients of the T basis [N1, N2, N3, R1, R2, R3] (basis.py:58-72):
+    the N (trace) blocks carry lam*C1, the R blocks mu*C1."""
+    nu = lam / (2.0 * (lam + mu))
+    one_minus_nu = (lam + 2.0 * mu) / (2.0 * (lam + mu))
+    C1 = 1.0 / (16.0 * np.pi * mu * one_minus_nu)
+    c34 = 3.0 - 4.0 * nu
+    cL = lam * C1
+    cM = mu * C1
+    blob = 6.0 * one_minus_nu
+    return np.array([c34 * cL, -cL, blob * cL, c34 * cM, -cM, blob * cM])
+
+
+def as_eps_array(eps, n_source: int) -> np.ndarray:
+    """Promote a scalar eps to (N_src,), validate length and positivity
+    (basis.py:187-204): every entry finite and > 0, else ValueError."""
+    arr = np.asarray(eps, dtype=float)
+    if arr.ndim == 0:
+        arr = np.full(n_source, float(arr))
+    elif arr.shape != (n_source,):
+        raise ValueError(f"eps array shape {arr.shape} != ({n_source},)")
+    bad = ~(np.isfinite(arr) & (arr > 0.0))
+    if bad.any():
+        raise ValueError(f"eps must be finite and > 0: {int(bad.sum())} bad "
+                         f"of {arr.size}, min = {arr.min()}")
+    return np.ascontiguousarray(arr)
+
+
+def element_sizes(mesh) -> np.ndarray:
+    """Per-triangle size h_j = mean edge length (basis.py:207-212)."""
+    tv = np.asarray(mesh.vertices, float)[np.asarray(mesh.triangles)]
+    e = np.stack([tv[:, 1] - tv[:, 0], tv[:, 2] - tv[:, 1], tv[:, 0] - tv[:, 2]])
+    return np.linalg.norm(e, axis=2).mean(axis=0)
+
+
+def resolve_eps(eps, mesh) -> np.ndarray:
+    """eps spec of one source mesh -> (N_src,) (basis.py:215-231): a scalar,
+    an (N_src,) array, or "auto" = EPS_OVER_H * h_j."""
+    if isinstance(eps, str):
+        if eps != "auto":
+            raise ValueError(f"unknown eps spec {eps!r} (only 'auto')")
+        return np.ascontiguousarray(defaults.EPS_OVER_H * element_sizes(mesh))
+    return as_eps_array(eps, mesh.n_triangles)
+
+
+def resolve_patch_eps(spec, patch) -> np.ndarray:
+    """eps of one patch or fault (basis.py:234-254): a dict is keyed by patch
+    name; "auto" on a fault is one value FAULT_EPS_OVER_H * min(h)."""
+    from ..model.core import FAULT_KINDS
+
+    if isinstance(spec, dict):
+        if patch.name not in spec:
+            raise ValueError(f"eps dict has no entry for patch "
+                             f"{patch.name!r} (keys: {sorted(spec)})")
+        spec = spec[patch.name]
+    if isinstance(spec, str) and spec == "auto" and patch.bc in FAULT_KINDS:
+        h = element_sizes(patch.mesh)
+        return np.full(h.size, defaults.FAULT_EPS_OVER_H * float(h.min()))
+    return resolve_eps(spec, patch.mesh)
+
+
+def _source_arrays(mesh_source):
+    tri_verts = np.asarray(mesh_source.vertices, float)[np.asarray(mesh_source.triangles)]
+    normals, _ = mesh_source.normals_and_areas()
+    return np.ascontiguousarray(tri_verts), np.ascontiguousarray(normals)
+
+
+def _field_points(mesh_or_points):
+    if hasattr(mesh_or_points, "centroids"):
+        return np.ascontiguousarray(mesh_or_points.centroids())
+    return np.ascontiguousarray(np.asarray(mesh_or_points, dtype=float))
+
+
+@dataclass
+class UBasis:
+    """Geometry-only U basis stack (3, 3N_f, 3N_s) (basis.py:304-314)."""
+    stack: np.ndarray
+
+    def combine(self, material=None, mu=None, lam=None) -> np.ndarray:
+        c = u_coeffs(*_mu_lam(material, mu, lam))
+        return np.tensordot(c, self.stack, axes=1)
+
+    def nbytes(self) -> int:
+        return self.stack.nbytes
+
+
+@dataclass
+class TBasis:
+    """Geometry-only T basis stack (6, 3N_f, 3N_s) (basis.py:317-327)."""
+    stack: np.ndarray
+
+    def combine(self, material=None, mu=None, lam=None) -> np.ndarray:
+        c = t_coeffs(*_mu_lam(material, mu, lam))
+        return np.tensordot(c, self.stack, axes=1)
+
+    def nbytes(self) -> int:
+        return self.stack.nbytes
+
+
+def _order0(order):
+    if order != 0:
+        raise NotImplementedError("only P0 (order 0) densities are built on the GPU path")
+
+
+def assemble_u_basis(mesh_field, mesh_source, eps, arrays=None, order: int = 0) -> UBasis:
+    """basis.py:336-342 (GPU kernel)."""
+    from . import tri_kernels as tk
+    _order0(order)
+    x_field = _field_points(mesh_field)
+    tri_verts, _ = _source_arrays(mesh_source)
+    return UBasis(tk.u_basis_matrices(x_field, tri_verts, as_eps_array(eps, tri_verts.shape[0])))
+
+
+def assemble_t_basis(mesh_field, mesh_source, eps, arrays=None, order: int = 0) -> TBasis:
+    """basis.py:345-351 (GPU kernel)."""
+    from . import tri_kernels as tk
+    _order0(order)
+    x_field = _field_points(mesh_field)
+    tri_verts, normals = _source_arrays(mesh_source)
+    return TBasis(tk.t_basis_matrices(x_field, tri_verts, normals,
+                                      as_eps_array(eps, tri_verts.shape[0])))
+
+
+def assemble_u_matrix(mesh_field, mesh_source, material, eps, arrays=None, order: int = 0,
+                      ids=None, n_cols=None) -> np.ndarray:
+    """One-shot U matrix (3N_f, 3N_s) (basis.py:372-386; P0, no shared ids)."""
+    from . import tri_kernels as tk
+    _order0(order)
+    if ids is not None:
+        raise NotImplementedError("shared-node columns (continuous layouts) are not in this build")
+    x_field = _field_points(mesh_field)
+    tri_verts, _ = _source_arrays(mesh_source)
+    g1, g2, g3 = u_coeffs(material.mu, material.lam)
+    return tk.u_matrix_direct(x_field, tri_verts, as_eps_array(eps, tri_verts.shape[0]),
+                              g1, g2, g3)
+
+
+def assemble_t_matrix(mesh_field, mesh_source, material, eps, arrays=None, order: int = 0,
+                      ids=None, n_cols=None) -> np.ndarray:
+    """One-shot T matrix (3N_f, 3N_s) (basis.py:389-403; P0, no shared ids)."""
+    from . import tri_kernels as tk
+    _order0(order)
+    if ids is not None:
+        raise NotImplementedError("shared-node columns (continuous layouts) are not in this build")
+    x_field = _field_points(mesh_field)
+    tri_verts, normals = _source_arrays(mesh_source)
+    c = t_coeffs(material.mu, material.lam)
+    return tk.t_matrix_direct(x_field, tri_verts, normals,
+                              as_eps_array(eps, tri_verts.shape[0]), *c)

@@ -1,0 +1,256 @@
+"""Flexible right-preconditioned GMRES on device-resident vectors
+(mbem's src/mbem/la/solver.py:115-245, the plain path).
+
+Same contract as the reference: ``fgmres(A, b, M, x0, rtol, restart, maxiter,
+stagnation_window, stagnation_factor, callback) -> (x, SolveReport)``, right
+preconditioning, Givens rotations in the reference's ``phase`` form, the
+``H[j+1, j] > 1e-300`` breakdown guard, Arnoldi residual ``|g[j+1]| / ||b||``,
+stagnation when no factor-``stagnation_factor`` drop in
+``stagnation_window`` iterations, ``x += Z^T y`` at each restart and the TRUE
+residual ``||b - A x|| / ||b||`` deciding ``converged``; ``||b|| = 0`` gives
+zeros.
+
+Where it differs, and why: the Krylov basis V (restart+1, n) and the
+flexible basis Z (restart, n) live in HBM and the orthogonalization is
+classical Gram-Schmidt applied twice (CGS2: two multi-dot + multi-axpy
+passes, each one launch over the whole basis) instead of modified
+Gram-Schmidt's j+1 dependent dot/axpy pairs -- the same projection in exact
+arithmetic and as stable as MGS in floating point, at two HBM passes per
+iteration.  The small Hessenberg least-squares problem stays on the host in
+FP64, exactly the reference's update order.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _lib, defaults
+from .._lib import call, ptr, stream
+
+F64 = torch.float64
+
+
+@dataclass
+class SolveReport:
+    """solver.py:31-68."""
+    converged: bool
+    iterations: int
+    true_relres: float
+    arnoldi_relres: float
+    stagnated: bool = False
+    restarts: int = 0
+    residual_history: list = field(default_factory=list)
+    precond_summary: dict | None = None
+    recycle_dim: int = 0
+    setup_matvecs: int = 0
+    matvecs: int = 0
+    precond_applies: int = 0
+
+    def __str__(self):
+        status = "converged" if self.converged else (
+            "STAGNATED" if self.stagnated else "NOT CONVERGED")
+        s = (f"FGMRES {status}: {self.iterations} iters, "
+             f"true relres {self.true_relres:.3e}")
+        if self.precond_summary:
+            counts: dict = {}
+            for sb in self.precond_summary["super_blocks"]:
+                counts[sb["rung"]] = counts.get(sb["rung"], 0) + 1
+            rungs = ", ".join(f"{n} {r}" for r, n in counts.items())
+            s += f"; preconditioner {rungs}, built in {self.precond_summary['build_s']:.2f} s"
+        return s
+
+
+def _solve_upper(R, g):
+    """Back substitution of the small upper-triangular system (solver.py:503-510)."""
+    m = R.shape[0]
+    y = np.zeros(m, dtype=R.dtype)
+    for i in range(m - 1, -1, -1):
+        s = g[i] - R[i, i + 1:] @ y[i + 1:]
+        y[i] = s / R[i, i]
+    return y
+
+
+def _as_op(A):
+    """Operator ``op(v, out)`` writing A v into ``out`` (solver.py:513-520):
+    a device matrix is applied with the library's GEMV, an object with
+    ``apply_into(v, out)`` writes in place, and ``.matvec`` objects or plain
+    callables return a new vector that is copied in."""
+    if A is None:
+        return None
+    if isinstance(A, torch.Tensor):
+        return lambda v, out: gemv(A, v, out)
+    if hasattr(A, "apply_into"):
+        return A.apply_into
+    f = A.matvec if hasattr(A, "matvec") else A
+    return lambda v, out: out.copy_(f(v))
+
+
+def gemv(A: torch.Tensor, x: torch.Tensor, out=None, alpha=1.0, beta=0.0):
+    """out = alpha A x + beta out on the device (mbem_gemv)."""
+    m, n = A.shape
+    if out is None:
+        out = torch.empty(m, dtype=F64, device=A.device)
+        beta = 0.0
+    call("mbem_gemv", ptr(A), A.stride(0), m, n, ptr(x), ptr(out), float(alpha), float(beta),
+         stream())
+    return out
+
+
+class _Krylov:
+    """Device workspace and the three vector passes of one FGMRES."""
+
+    def __init__(self, n, m, dev):
+        self.n = n
+        self.V = torch.empty((m + 1, n), dtype=F64, device=dev)
+        self.Z = torch.empty((m, n), dtype=F64, device=dev)
+        self.h = torch.empty(m + 1, dtype=F64, device=dev)
+        self.work = torch.empty(int(_lib.load().mbem_multidot_work(m + 1, n)), dtype=F64,
+                                device=dev)
+
+    def dots(self, rows: torch.Tensor, k: int, w: torch.Tensor) -> torch.Tensor:
+        call("mbem_multidot", ptr(rows), rows.stride(0), k, self.n, ptr(w), ptr(self.h),
+             ptr(self.work), stream())
+        return self.h[:k]
+
+    def axpy_rows(self, rows: torch.Tensor, k: int, coef: torch.Tensor, w: torch.Tensor,
+                  alpha: float):
+        call("mbem_multiaxpy", ptr(rows), rows.stride(0), k, self.n, ptr(coef), ptr(w),
+             float(alpha), stream())
+
+    def norm(self, w: torch.Tensor) -> float:
+        d = self.dots(w.view(1, -1), 1, w)
+        return float(np.sqrt(max(float(d.item()), 0.0)))
+
+
+def axpby(a, x, b, y):
+    call("mbem_axpby", y.numel(), float(a), ptr(x), float(b), ptr(y), stream())
+    return y
+
+
+def fgmres(A, b, M=None, x0=None,
+           rtol: float = defaults.GMRES_RTOL,
+           restart: int = defaults.GMRES_RESTART,
+           maxiter: int = defaults.GMRES_MAXITER,
+           stagnation_window: int = defaults.GMRES_STAGNATION_WINDOW,
+           stagnation_factor: float = defaults.GMRES_STAGNATION_FACTOR,
+           callback=None, recycle=None):
+    """Solve A x = b on the device; returns (x device tensor, SolveReport).
+
+    ``A`` and ``M`` are device matrices or callables ``f(v) -> A v`` on
+    device vectors; ``M`` approximates A^-1 (right preconditioner)."""
+    if recycle is not None:
+        raise NotImplementedError("GCRO-DR recycling is not in this build")
+    _lib.require_gpu()
+    A_mv = _as_op(A)
+    M_mv = _as_op(M) if M is not None else None
+    b = b if isinstance(b, torch.Tensor) else torch.as_tensor(np.asarray(b, float))
+    b = b.to(device=_lib.device(), dtype=F64).contiguous()
+    n = b.shape[0]
+    dev = b.device
+    ws = _Krylov(n, max(1, min(restart, maxiter)), dev)
+    b_norm = ws.norm(b)
+    if b_norm == 0.0:
+        return torch.zeros(n, dtype=F64, device=dev), SolveReport(True, 0, 0.0, 0.0)
+    x = torch.zeros(n, dtype=F64, device=dev) if x0 is None else \
+        torch.as_tensor(x0, dtype=F64, device=dev).clone().contiguous()
+
+    history: list = []
+    total_iters = 0
+    restarts = 0
+    best_res = np.inf
+    best_res_at = 0
+    stagnated = False
+    n_mv = 0
+    n_pc = 0
+    r = torch.empty(n, dtype=F64, device=dev)
+
+    while total_iters < maxiter and not stagnated:
+        A_mv(x, r)
+        n_mv += 1
+        axpby(1.0, b, -1.0, r)                       # r = b - A x
+        beta = ws.norm(r)
+        arnoldi_res = beta / b_norm
+        if arnoldi_res < rtol:
+            break
+        m = min(restart, maxiter - total_iters)
+        V, Z = ws.V, ws.Z
+        H = np.zeros((m + 1, m))
+        cs = np.zeros(m)
+        sn = np.zeros(m)
+        g = np.zeros(m + 1)
+        axpby(1.0 / beta, r, 0.0, V[0])
+        g[0] = beta
+        j_done = 0
+        for j in range(m):
+            if M_mv is None:
+                Z[j].copy_(V[j])
+            else:
+                M_mv(V[j], Z[j])
+                n_pc += 1
+            w = V[j + 1]
+            A_mv(Z[j], w)
+            n_mv += 1
+            # classical Gram-Schmidt, twice
+            h1 = ws.dots(V, j + 1, w)
+            ws.axpy_rows(V, j + 1, h1, w, -1.0)
+            hsum = h1.cpu().numpy().copy()
+            h2 = ws.dots(V, j + 1, w)
+            ws.axpy_rows(V, j + 1, h2, w, -1.0)
+            hsum += h2.cpu().numpy()
+            H[:j + 1, j] = hsum
+            H[j + 1, j] = ws.norm(w)
+            if H[j + 1, j] > 1e-300:
+                axpby(1.0 / H[j + 1, j], w, 0.0, w)
+            # Givens (solver.py:187-205)
+            for i in range(j):
+                t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                H[i, j] = t
+            denom = np.sqrt(np.abs(H[j, j]) ** 2 + np.abs(H[j + 1, j]) ** 2)
+            if denom == 0.0:
+                j_done = j + 1
+                break
+            cs[j] = np.abs(H[j, j]) / denom if np.abs(H[j, j]) > 0 else 0.0
+            if np.abs(H[j, j]) > 0:
+                phase = H[j, j] / np.abs(H[j, j])
+                sn[j] = phase * H[j + 1, j] / denom
+            else:
+                sn[j] = 1.0
+            H[j, j] = cs[j] * H[j, j] + sn[j] * H[j + 1, j]
+            H[j + 1, j] = 0.0
+            g[j + 1] = -sn[j] * g[j]
+            g[j] = cs[j] * g[j]
+            total_iters += 1
+            j_done = j + 1
+            arnoldi_res = float(np.abs(g[j + 1])) / b_norm
+            history.append(arnoldi_res)
+            if callback is not None:
+                callback(total_iters, arnoldi_res)
+            if arnoldi_res < best_res / stagnation_factor:
+                best_res = arnoldi_res
+                best_res_at = total_iters
+            elif total_iters - best_res_at >= stagnation_window:
+                stagnated = True
+                break
+            if arnoldi_res < rtol or total_iters >= maxiter:
+                break
+        if j_done > 0:
+            y = _solve_upper(H[:j_done, :j_done], g[:j_done])
+            yd = torch.as_tensor(y, dtype=F64, device=dev)
+            ws.axpy_rows(Z, j_done, yd, x, 1.0)
+        restarts += 1
+        if stagnated:
+            break
+
+    A_mv(x, r)
+    n_mv += 1
+    axpby(1.0, b, -1.0, r)
+    true_relres = ws.norm(r) / b_norm
+    arnoldi_final = history[-1] if history else true_relres
+    converged = true_relres < rtol
+    return x, SolveReport(converged=converged, iterations=total_iters, true_relres=true_relres,
+                          arnoldi_relres=float(arnoldi_final),
+                          stagnated=stagnated and not converged, restarts=restarts - 1,
+                          residual_history=history, matvecs=n_mv, precond_applies=n_pc)
