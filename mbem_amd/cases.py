@@ -1,0 +1,87 @@
+"""The box-with-dipping-fault workloads of the dense path.
+
+A 40 x 40 x 20 km box (x, y in [-20, 20], z in [-20, 0]) meshed on a
+structured grid with ``n_cells`` cells per 40 km on every face (the
+symmetric-diagonal grid of src/local_box_mesh.py:38-91): top and base
+2 n^2 triangles each, each wall n x n/2 cells.  Top: FREE_TRACTION (zero);
+sides and base: PRESCRIBED_DISPLACEMENT (zero), so the model is anchored.
+A 10 x 10 km planar fault, meshed vertically (local_box_mesh.py:241-290)
+and turned about its trace to the dip (cases/layered_box.py:59-71), centred
+at x = y = 0 with its top edge at z = -3 km, carries uniform unit reverse
+slip: the Burgers vector is the up-dip unit vector (the hanging wall, on the
++n side, moves up dip).  eps = 0.5 x the patch's mean edge length, one value
+per patch.  Units: km, GPa (mu = 30 GPa = 3e10 Pa, nu = 0.25 -> lam = 30).
+
+n_cells = 16 gives N = 2,048 box triangles (n = 6,144 unknowns), n_cells =
+50 gives N = 20,000 (n = 60,000; the FP64 matrix is 28.8 GB), n_cells = 35
+gives N = 9,800 (n = 29,400).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .kernels.basis import element_sizes
+from .mesh import (ElasticMaterial, concatenate, rectangular_patch, tilted_fault,
+                   vertical_fault, vertical_panel)
+from .model import BCType, Patch, Region, RegionModel
+
+
+def box_meshes(n_cells: int, half=20.0, depth=20.0):
+    nz = max(1, n_cells // 2)
+    xr = (-half, half)
+    top = rectangular_patch(xr, xr, 0.0, n_cells, n_cells, normal_up=True)
+    base = rectangular_patch(xr, xr, -depth, n_cells, n_cells, normal_up=False)
+    sides = concatenate([
+        vertical_panel("x", half, xr, (-depth, 0.0), n_cells, nz, +1),
+        vertical_panel("x", -half, xr, (-depth, 0.0), n_cells, nz, -1),
+        vertical_panel("y", half, xr, (-depth, 0.0), n_cells, nz, +1),
+        vertical_panel("y", -half, xr, (-depth, 0.0), n_cells, nz, -1)])
+    return top, sides, base
+
+
+def dipping_fault(size=10.0, cells=(10, 20), dip=45.0, top_z=-3.0):
+    """(mesh, normal, burgers): the fault plane dipping toward +x."""
+    vert, _, _ = vertical_fault(size, (-size, 0.0), cells[0], cells[1])
+    mesh, normal = tilted_fault(vert, dip)
+    v = mesh.vertices.copy()
+    xs = v[:, 0]
+    v[:, 0] -= 0.5 * (xs.min() + xs.max())
+    v[:, 2] += top_z - v[:, 2].max()
+    mesh.vertices = v
+    d = np.deg2rad(dip)
+    burgers = np.array([-np.cos(d), 0.0, np.sin(d)])
+    return mesh, normal, burgers
+
+
+def box_fault_model(n_cells: int = 16, fault_cells=(10, 20), dip: float = 45.0,
+                    mu: float = 30.0, lam: float = 30.0, eps_over_h: float = 0.5,
+                    slip: float = 1.0, fault: bool = True):
+    """(model, eps spec, info) of the box-with-fault workload."""
+    top, sides, base = box_meshes(n_cells)
+    p_top = Patch("top", top, BCType.FREE_TRACTION)
+    p_sides = Patch("sides", sides, BCType.PRESCRIBED_DISPLACEMENT)
+    p_base = Patch("base", base, BCType.PRESCRIBED_DISPLACEMENT)
+    faults = []
+    eps = {}
+    for p in (p_top, p_sides, p_base):
+        eps[p.name] = eps_over_h * float(element_sizes(p.mesh).mean())
+    info = {"n_cells": n_cells, "box_triangles": top.n_triangles + sides.n_triangles
+            + base.n_triangles}
+    if fault:
+        fmesh, fnormal, burgers = dipping_fault(cells=fault_cells, dip=dip)
+        p_fault = Patch("fault", fmesh, BCType.FAULT,
+                        value=np.broadcast_to(slip * burgers, (fmesh.n_triangles, 3)))
+        faults.append(p_fault)
+        eps["fault"] = eps_over_h * float(element_sizes(fmesh).mean())
+        info.update(fault_triangles=fmesh.n_triangles, fault_normal=fnormal.tolist(),
+                    burgers=burgers.tolist())
+    region = Region("crust", ElasticMaterial(mu=mu, lam=lam), [p_top, p_sides, p_base],
+                    probe_point=np.array([9.0, 9.0, -10.0]), faults=faults)
+    return RegionModel([region]), eps, info
+
+
+def surface_grid(n=50, half=20.0, z=0.0):
+    """An n x n grid of points over [-half, half]^2 at height z."""
+    g = np.linspace(-half, half, n)
+    X, Y = np.meshgrid(g, g, indexing="xy")
+    return np.column_stack([X.ravel(), Y.ravel(), np.full(X.size, z)])

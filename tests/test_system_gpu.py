@@ -1,0 +1,141 @@
+"""GPU parity of the dense system, its solves and the readout against the
+oracle, on the CPU-runnable box-with-dipping-fault workload (N = 2,048 box
+triangles, n = 6,144 unknowns, 400 fault triangles, eps = 0.5 h).
+
+Bars (FP64, north star 1e-10 relative):
+  A, b, calibrated diagonal : every entry within 1e-10 of max|.|, Frobenius
+                              within 1e-12;
+  LU solution               : within 1e-10 (relative 2-norm) of the oracle's LU;
+  FGMRES solution           : converged to true relres 1e-13, then within
+                              1e-10 of the oracle's LU, iteration count
+                              within 2 of the oracle's MGS FGMRES;
+  displacement / stress     : within 1e-10 of the oracle readout on the
+                              50 x 50 surface grid and an interior grid.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def c1():
+    from mbem_amd.backends.dense import DenseBackend
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.model import generate_system
+    from oracle import dense as od
+    model, eps, info = box_fault_model(16)
+    system = generate_system(model)
+    print(f"case: box n_cells=16, n={system.layout.n_unknowns}", flush=True)
+    asm = DenseBackend(calibrate_all_rows=True).assemble(system, eps)
+    A, b, calib = od.assemble(system, eps)
+    x_lu = od.lu_solve(A, b)
+    return dict(model=model, eps=eps, system=system, asm=asm, A=A, b=b, calib=calib, x_lu=x_lu)
+
+
+def _rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def test_assembled_matrix_and_rhs(c1):
+    asm = c1["asm"]
+    A, b = asm.A, asm.b
+    err = np.max(np.abs(A - c1["A"])) / np.max(np.abs(c1["A"]))
+    print(f"A: max entry err {err:.2e}, fro {_rel(A, c1['A']):.2e}; "
+          f"b: {_rel(b, c1['b']):.2e}; assembly {asm.assembly.seconds:.3f} s", flush=True)
+    assert err < 1e-10
+    assert _rel(A, c1["A"]) < 1e-12
+    assert np.max(np.abs(b - c1["b"])) < 1e-10 * np.max(np.abs(c1["b"]))
+    # per slot block, so no block hides under a larger one
+    for rs in asm.layout.slots:
+        for cs in asm.layout.slots:
+            blk = A[rs.offset:rs.stop, cs.offset:cs.stop]
+            ref = c1["A"][rs.offset:rs.stop, cs.offset:cs.stop]
+            assert np.max(np.abs(blk - ref)) <= 1e-10 * max(np.max(np.abs(ref)), 1e-300)
+
+
+def test_calibrated_diagonal(c1):
+    asm = c1["asm"]
+    for key, C in c1["calib"].items():
+        G = asm.calib[key]
+        assert np.max(np.abs(G - C)) < 1e-10 * np.max(np.abs(C))
+
+
+def test_lu_solve(c1):
+    asm = c1["asm"]
+    sol = asm.solve(method="lu")
+    x = np.concatenate([sol[s.name].ravel() for s in asm.layout.slots])
+    print(f"LU: rel err vs oracle LU {_rel(x, c1['x_lu']):.2e}, residual {asm.report.residual:.2e}")
+    assert _rel(x, c1["x_lu"]) < 1e-10
+    asm.release()
+
+
+def test_fgmres_block_jacobi(c1):
+    from oracle import dense as od
+    asm = c1["asm"]
+    sol = asm.solve(rtol=1e-13, precond_max_dense=0)
+    rep = asm.report
+    x = np.concatenate([sol[s.name].ravel() for s in asm.layout.slots])
+    print(f"FGMRES: {rep}; rel err vs oracle LU {_rel(x, c1['x_lu']):.2e}", flush=True)
+    assert rep.converged and rep.true_relres < 1e-13
+    assert _rel(x, c1["x_lu"]) < 1e-10
+    pts = {id(p): p.collocation_points() for p in c1["model"].regions[0].patches}
+    A = c1["A"]
+    M = od.BlockJacobiGS(A, asm.layout, lambda p: pts[id(p)], max_dense=0)
+    _, orep = od.fgmres(lambda v: A @ v, c1["b"], M=M, rtol=1e-10)
+    asm.release()
+    asm.solve(rtol=1e-10, precond_max_dense=0)
+    print(f"iterations to 1e-10: GPU {asm.report.iterations}, oracle {orep['iterations']}")
+    assert abs(asm.report.iterations - orep["iterations"]) <= 2
+
+
+def test_readout(c1):
+    from mbem_amd.cases import surface_grid
+    from mbem_amd.evaluate import evaluate_displacement, evaluate_stress
+    from oracle import evaluate as oe
+    from oracle.dense import split
+    model = c1["model"]
+    region = model.regions[0]
+    sol = split(c1["system"].layout, c1["x_lu"])
+    eps = c1["eps"]
+    grid = surface_grid(50)
+    inner = surface_grid(12, half=15.0, z=-4.0 * eps["top"])
+    pts = np.vstack([grid, inner])
+    u = evaluate_displacement(model, region, sol, pts, eps, warn_near=False)
+    ur = oe.evaluate_displacement(model, region, sol, pts, eps)
+    print(f"displacement rel err {_rel(u, ur):.2e}")
+    assert np.max(np.abs(u - ur)) < 1e-10 * np.max(np.abs(ur))
+    s = evaluate_stress(model, region, sol, pts, eps, warn_near=False)
+    sr = oe.evaluate_stress(model, region, sol, pts, eps)
+    print(f"stress rel err {_rel(s, sr):.2e}")
+    assert np.max(np.abs(s - sr)) < 1e-10 * np.max(np.abs(sr))
+    s_full = evaluate_stress(model, region, sol, pts, eps, warn_near=False, near_list=False)
+    sr_full = oe.evaluate_stress(model, region, sol, pts, eps, near_list=False)
+    assert np.max(np.abs(s_full - sr_full)) < 1e-10 * np.max(np.abs(sr_full))
+
+
+def test_rigid_translation_gives_zero_traction():
+    """Manufactured solution (verify_solved_bvp.py B1): prescribe a constant
+    displacement on every wall and the base, leave the top free: the solve
+    returns u_top = c and zero traction on the walls, to rounding -- the
+    calibrated diagonal annihilates constants exactly."""
+    from mbem_amd.backends.dense import DenseBackend
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.model import generate_system
+    c = np.array([0.3, -0.2, 0.5])
+    model, eps, _ = box_fault_model(8, fault=False)
+    for p in model.regions[0].patches[1:]:
+        p.value = c
+    asm = DenseBackend().assemble(generate_system(model), eps)
+    sol = asm.solve(method="lu")
+    assert np.max(np.abs(sol["u:top"] - c)) < 1e-10
+    assert np.max(np.abs(sol["t:sides"])) < 1e-9 * 30.0
+    assert np.max(np.abs(sol["t:base"])) < 1e-9 * 30.0
+
+
+def test_fault_convention_selfcheck():
+    from mbem_amd import selfcheck
+    selfcheck.ensure_fault_convention("stress")
+    rep = selfcheck.report()
+    print(rep)
+    assert set(rep) == {"core", "stress"}
