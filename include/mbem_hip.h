@@ -139,6 +139,12 @@ int mbem_gather_block(const double *A, int64_t lda, const int64_t *idx, int64_t 
  * lu_factor), kept as an explicit inverse so the apply is one GEMV. */
 int mbem_invert(double *M, int64_t m, int32_t *ipiv, int32_t *info, hipStream_t stream);
 
+/* In-place inverses of `batch` equal-size m x m chunks stored back to back
+ * (stride m*m), one strided-batched getrf + getri; ipiv batch*m int32, info
+ * batch int32 (device). */
+int mbem_invert_batched(double *M, int64_t m, int64_t batch, int32_t *ipiv, int32_t *info,
+                        hipStream_t stream);
+
 /* Dense LU of a row-major n x n operator in place and the solve A x = b:
  * the reference dense solve (backends/dense.py:532-571, la/lu.py:110-239 with
  * lu="double"). */

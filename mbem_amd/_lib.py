@@ -44,6 +44,7 @@ _SIGS = {
     "mbem_axpby": [_i64, _d, _p, _d, _p, _p],
     "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert": [_p, _i64, _p, _p, _p],
+    "mbem_invert_batched": [_p, _i64, _i64, _p, _p, _p],
     "mbem_lu_factor": [_p, _i64, _p, _p, _p],
     "mbem_lu_solve": [_p, _i64, _p, _p, _i64, _p],
     "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],

@@ -110,7 +110,14 @@ def warn_collocation_near_fault(system: BlockSystem, eps):
                     eps_f = kb.resolve_patch_eps(eps, f)
                 except ValueError:
                     continue
-                d, idx = distance_to_mesh(x, f.mesh)
+                # only points within reach of the fault's bounding box can be
+                # close: the exact test runs on those (same result, less work)
+                fv = np.asarray(f.mesh.vertices, float)
+                gap = np.maximum(0.0, np.maximum(fv.min(axis=0) - x, x - fv.max(axis=0)))
+                cand = np.flatnonzero(np.linalg.norm(gap, axis=1) < limit * float(eps_f.max()))
+                if cand.size == 0:
+                    continue
+                d, idx = distance_to_mesh(x[cand], f.mesh)
                 ratio = d / eps_f[idx]
                 close = int(np.sum(ratio < limit))
                 if close:
@@ -166,9 +173,11 @@ class AssembledDense:
         if fault_drag:
             raise NotImplementedError("fault drag needs FAULT_TRACTION faults (not in this build)")
         _lib.require_gpu()
+        t0 = time.perf_counter()
         require_anchor_or_deflate(system, jump, deflate)
         warn_half_jump_eps(system, eps, jump)
         warn_collocation_near_fault(system, eps)
+        self.timings = {"checks_s": time.perf_counter() - t0}
         self.system = system
         self.layout = system.layout
         self.eps = eps
@@ -188,8 +197,11 @@ class AssembledDense:
         self._b_host = None
         self.A_dev = None
         self.b_dev = None
+        t0 = time.perf_counter()
         self._prepare_geometry()
+        self.timings["geometry_s"] = time.perf_counter() - t0
         self._build()
+        self.timings["assembly_s"] = self.assembly.seconds
 
     # -- geometry on the device ------------------------------------------
     def _prepare_geometry(self):
@@ -410,7 +422,10 @@ class AssembledDense:
     def _solve_fgmres(self, rtol, restart, maxiter, x0, max_dense, above_dense, bj_chunk,
                       recycle):
         from ..la.solver import fgmres
+        t0 = time.perf_counter()
         M = self.preconditioner(max_dense, bj_chunk, above_dense)
+        self.timings["precond_s"] = time.perf_counter() - t0
+        t0 = time.perf_counter()
         if x0 is not None and not isinstance(x0, torch.Tensor):
             x0 = torch.as_tensor(np.asarray(x0, float), dtype=F64, device=self.device)
         if self.deflate:
@@ -433,6 +448,8 @@ class AssembledDense:
         else:
             x, report = fgmres(self, self.b_dev, M=M, x0=x0, rtol=rtol, restart=restart,
                                maxiter=maxiter, recycle=recycle)
+        torch.cuda.synchronize()
+        self.timings["fgmres_s"] = time.perf_counter() - t0
         report.precond_summary = M.summary()
         self.report = report
         return x

@@ -37,6 +37,10 @@ __global__ __launch_bounds__(256) void k_prepare_src(const double* __restrict__ 
                                                      const double* __restrict__ eps, long ns,
                                                      double* __restrict__ src, long ld)
 {
+    // no FMA contraction here: the degenerate test (area2 < 1e-30) relies on
+    // the cross product of parallel edges being exactly zero, as it is in the
+    // reference's uncontracted arithmetic
+#pragma clang fp contract(off)
     long s = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= ns) return;
     double v[9];

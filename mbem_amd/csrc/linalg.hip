@@ -266,6 +266,29 @@ int mbem_invert(double* M, int64_t m, int32_t* ipiv, int32_t* info, hipStream_t 
     return launch_status();
 }
 
+// In-place inverses of `batch` m x m matrices stored back to back (stride
+// m*m): one strided-batched getrf + getri, so the per-column panel kernels
+// run once for the whole batch.  ipiv: batch*m int32, info: batch int32.
+int mbem_invert_batched(double* M, int64_t m, int64_t batch, int32_t* ipiv, int32_t* info,
+                        hipStream_t stream)
+{
+    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info)) || m > INT32_MAX
+        || batch > INT32_MAX)
+        return MBEM_EINVAL;
+    if (m == 0 || batch == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    rocblas_int mm = (rocblas_int)m;
+    rocblas_stride sA = (rocblas_stride)(m * m), sP = (rocblas_stride)m;
+    if (rocsolver_dgetrf_strided_batched(g_handle, mm, mm, M, mm, sA, ipiv, sP, info,
+                                         (rocblas_int)batch) != rocblas_status_success)
+        return MBEM_EHIP;
+    if (rocsolver_dgetri_strided_batched(g_handle, mm, M, mm, sA, ipiv, sP, info,
+                                         (rocblas_int)batch) != rocblas_status_success)
+        return MBEM_EHIP;
+    return launch_status();
+}
+
 // LU factorization of a row-major n x n matrix in place (LAPACK getrf of its
 // column-major view, i.e. of A^T), and the matching solve A x = b (getrs with
 // transpose on that factor).  ipiv: n int32, info: 1 int32, both on device.

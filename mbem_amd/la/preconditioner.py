@@ -193,9 +193,6 @@ class BlockGaussSeidel:
         self.inv_ptr = torch.as_tensor(iptr, dtype=I64, device=dev)
         self.idx = torch.as_tensor(np.concatenate([c[1] for c in chunks]), dtype=I64, device=dev)
         self.inv = torch.empty(int(iptr[-1]), dtype=F64, device=dev)
-        mmax = int(sizes.max()) if sizes.size else 0
-        ipiv = torch.empty(max(mmax, 1), dtype=torch.int32, device=dev)
-        info = torch.zeros(1, dtype=torch.int32, device=dev)
         R = _lib.BJ_ROWS_PER_WAVE
         tables = {k: [] for k in range(len(self.sbs))}
         for ci, (k, dofs) in enumerate(chunks):
@@ -203,11 +200,24 @@ class BlockGaussSeidel:
             M = self.inv[int(iptr[ci]):int(iptr[ci + 1])]
             call("mbem_gather_block", ptr(A), A.stride(0),
                  ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), m, ptr(M), stream())
-            call("mbem_invert", ptr(M), m, ptr(ipiv), ptr(info), stream())
-            if int(info.item()) != 0:
-                raise np.linalg.LinAlgError(f"block-Jacobi chunk {ci} ({m} DOFs) is singular "
-                                            f"(getrf info {int(info.item())})")
             tables[k] += [(ci, r0) for r0 in range(0, m, R)]
+        # invert runs of equal-size chunks (contiguous in ``inv``) as one batch
+        ci = 0
+        while ci < len(chunks):
+            cj = ci
+            while cj + 1 < len(chunks) and sizes[cj + 1] == sizes[ci]:
+                cj += 1
+            m, nb = int(sizes[ci]), cj - ci + 1
+            ipiv = torch.empty(nb * m, dtype=torch.int32, device=dev)
+            info = torch.zeros(nb, dtype=torch.int32, device=dev)
+            call("mbem_invert_batched", ptr(self.inv[int(iptr[ci]):]), m, nb, ptr(ipiv),
+                 ptr(info), stream())
+            bad = info.cpu().numpy()
+            if np.any(bad != 0):
+                k = int(np.flatnonzero(bad)[0])
+                raise np.linalg.LinAlgError(f"block-Jacobi chunk {ci + k} ({m} DOFs) is singular "
+                                            f"(getrf info {int(bad[k])})")
+            ci = cj + 1
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
             sb.wave_table = torch.as_tensor(wt, dtype=I64, device=dev).contiguous()

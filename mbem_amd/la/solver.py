@@ -105,14 +105,27 @@ class _Krylov:
         self.n = n
         self.V = torch.empty((m + 1, n), dtype=F64, device=dev)
         self.Z = torch.empty((m, n), dtype=F64, device=dev)
-        self.h = torch.empty(m + 1, dtype=F64, device=dev)
+        self.h = torch.empty(2 * (m + 1) + 1, dtype=F64, device=dev)
         self.work = torch.empty(int(_lib.load().mbem_multidot_work(m + 1, n)), dtype=F64,
                                 device=dev)
 
-    def dots(self, rows: torch.Tensor, k: int, w: torch.Tensor) -> torch.Tensor:
-        call("mbem_multidot", ptr(rows), rows.stride(0), k, self.n, ptr(w), ptr(self.h),
+    def dots(self, rows: torch.Tensor, k: int, w: torch.Tensor, at: int = 0) -> torch.Tensor:
+        """h[at:at+k] = rows[:k] . w on the device (no host sync)."""
+        out = self.h[at:at + k]
+        call("mbem_multidot", ptr(rows), rows.stride(0), k, self.n, ptr(w), ptr(out),
              ptr(self.work), stream())
-        return self.h[:k]
+        return out
+
+    def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor):
+        """Project w against V[:k] twice (classical Gram-Schmidt, reorthogonalized)
+        and return (h, ||w||) on the host after ONE synchronization."""
+        h1 = self.dots(V, k, w, 0)
+        self.axpy_rows(V, k, h1, w, -1.0)
+        h2 = self.dots(V, k, w, k)
+        self.axpy_rows(V, k, h2, w, -1.0)
+        self.dots(w.view(1, -1), 1, w, 2 * k)
+        hv = self.h[:2 * k + 1].cpu().numpy()
+        return hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0)))
 
     def axpy_rows(self, rows: torch.Tensor, k: int, coef: torch.Tensor, w: torch.Tensor,
                   alpha: float):
@@ -192,15 +205,10 @@ def fgmres(A, b, M=None, x0=None,
             w = V[j + 1]
             A_mv(Z[j], w)
             n_mv += 1
-            # classical Gram-Schmidt, twice
-            h1 = ws.dots(V, j + 1, w)
-            ws.axpy_rows(V, j + 1, h1, w, -1.0)
-            hsum = h1.cpu().numpy().copy()
-            h2 = ws.dots(V, j + 1, w)
-            ws.axpy_rows(V, j + 1, h2, w, -1.0)
-            hsum += h2.cpu().numpy()
+            # classical Gram-Schmidt, twice; one host synchronization
+            hsum, wnorm = ws.cgs2(V, j + 1, w)
             H[:j + 1, j] = hsum
-            H[j + 1, j] = ws.norm(w)
+            H[j + 1, j] = wnorm
             if H[j + 1, j] > 1e-300:
                 axpby(1.0 / H[j + 1, j], w, 0.0, w)
             # Givens (solver.py:187-205)
