@@ -159,8 +159,14 @@ def main():
     ap.add_argument("--fault-cells", type=int, nargs=2, default=(13, 26))
     ap.add_argument("--target-err", type=float, default=1e-8)
     ap.add_argument("--rtol", type=float, default=2e-11)
-    ap.add_argument("--bj-chunk", type=int, default=None)
+    ap.add_argument("--bj-chunk", type=int, default=3000)
+    ap.add_argument("--grouping", choices=("patch", "region"), default="region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--method", choices=("plain", "mixed", "both"), default="both")
+    ap.add_argument("--storage", choices=("bf16", "fp32"), default="bf16")
+    ap.add_argument("--inner-rtol", type=float, default=5e-2)
+    ap.add_argument("--bj-chunk-mixed", type=int, default=3000)
+    ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
     ap.add_argument("--probe-traffic", type=int, default=0, metavar="N",
                     help="only run the matvec / copy traffic probe on an N x N matrix")
     args = ap.parse_args()
@@ -187,7 +193,7 @@ def main():
     def step():
         asm = AssembledDense(system, eps, "direct", jump="calibrated")
         x = asm.solve(rtol=args.rtol, precond_max_dense=0, precond_bj_chunk=args.bj_chunk,
-                      return_device=True)
+                      return_device=True, precond_grouping=args.grouping)
         return asm, x
 
     asm = None
@@ -214,28 +220,58 @@ def main():
     print(f"[bench] gemv {gemv_ms:.3f} ms = {gemv_bytes / gemv_ms / 1e6:.0f} GB/s setup "
           f"{time.perf_counter() - t_setup:.1f} s", flush=True)
 
-    times, errs, iters, asm_s = [], [], [], []
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        ts = time.perf_counter()
-        asm = None
-        asm, x = step()
+    def timed(step_fn, label):
+        times, errs, iters = [], [], []
+        asm_k = None
         torch.cuda.synchronize()
-        times.append(time.perf_counter() - ts)
-        errs.append(float(torch.linalg.vector_norm(x - x_ref) / torch.linalg.vector_norm(x_ref)))
-        iters.append(asm.report.iterations)
-        asm_s.append(asm.assembly.seconds)
-        ph = ", ".join(f"{k_}={v:.3f}" for k_, v in asm.timings.items())
-        print(f"[bench] step {k}: {times[-1]:.3f} s ({ph}; "
-              f"{asm.report.iterations} iterations, rel err {errs[-1]:.2e})", flush=True)
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
+        for k in range(args.steps):
+            asm_k = None
+            torch.cuda.empty_cache()
+            ts = time.perf_counter()
+            asm_k, x, rep = step_fn()
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - ts)
+            errs.append(float(torch.linalg.vector_norm(x - x_ref)
+                              / torch.linalg.vector_norm(x_ref)))
+            iters.append(rep.iterations)
+            ph = ", ".join(f"{k_}={v:.3f}" for k_, v in asm_k.timings.items())
+            print(f"[bench] {label} step {k}: {times[-1]:.3f} s ({ph}; {rep}; "
+                  f"rel err {errs[-1]:.2e})", flush=True)
+        return times, errs, iters, asm_k, rep
+
+    def step_plain():
+        a, x = step()
+        return a, x, a.report
+
+    def step_mixed():
+        from mbem_amd.la.mixed import solve_mixed
+        a = AssembledDense(system, eps, "direct", jump="calibrated")
+        t0 = time.perf_counter()
+        x, rep = solve_mixed(a, rtol=args.rtol, storage=args.storage,
+                             inner_rtol=args.inner_rtol, precond_bj_chunk=args.bj_chunk_mixed,
+                             precond_grouping=args.grouping_mixed)
+        a.timings.update(mixed_operator_s=rep.operator_build_s,
+                         precond_s=rep.precond_build_s, fgmres_s=rep.solve_s)
+        return a, x, rep
+
+    results = {}
+    if args.method in ("mixed", "both"):
+        step_mixed()          # warm the mixed kernels and rocSOLVER's FP32 path
+        torch.cuda.synchronize()
+    for label, fn in (("plain", step_plain), ("mixed", step_mixed)):
+        if args.method in (label, "both"):
+            results[label] = timed(fn, label)
+    main_label = "mixed" if "mixed" in results else "plain"
+    times, errs, iters, asm, rep_main = results[main_label]
+    total = float(np.sum(times))
     ms_per_step = 1e3 * total / args.steps
-    pre = asm.report.precond_summary
+    plain_s = float(np.mean(results["plain"][0])) if "plain" in results else None
+    mixed_s = float(np.mean(results["mixed"][0])) if "mixed" in results else None
+    pre = results["plain"][4].precond_summary if "plain" in results else {}
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(model, eps, system, int(np.median(iters)))
+        it_plain = results["plain"][2] if "plain" in results else [60]
+        cpu = cpu_baseline(model, eps, system, int(np.median(it_plain)))
         print(f"[bench] cpu baseline {cpu['value']:.1f} s on {cpu['cores']} cores", flush=True)
     ratio, src_name = _measured_traffic_ratio()
     line = {
@@ -255,12 +291,21 @@ def main():
             "workload": f"box 40x40x20 km, N={info['box_triangles']} P0 triangles "
                         f"(n={n}), dipping fault {info['fault_triangles']} triangles, "
                         f"eps=0.5h, mu=30 GPa nu=0.25",
-            "method": "plain FP64: GPU assembly + block-Jacobi FGMRES",
+            "method": ("mixed: BF16 far field + FP64 near field on an FP64 outer FGMRES, "
+                       "FP32 block-Jacobi" if main_label == "mixed" else
+                       "plain FP64: GPU assembly + block-Jacobi FGMRES"),
+            "storage": args.storage if main_label == "mixed" else "fp64",
             "rtol": args.rtol, "rel_err_vs_fp64_reference": max(errs),
             "target_rel_err": args.target_err, "iterations": iters,
-            "assembly_s": asm_s, "precond_build_s": pre["build_s"],
-            "precond_chunks": pre["chunks"],
-            "mixed_learned": "not built yet",
+            "plain_fp64_solve_s": plain_s, "mixed_solve_s": mixed_s,
+            "speedup_vs_plain_fp64": (plain_s / mixed_s if plain_s and mixed_s else None),
+            "plain_rel_err": (max(results["plain"][1]) if "plain" in results else None),
+            "precond_build_s": pre.get("build_s"), "precond_chunks": pre.get("chunks"),
+            "plain_precond": f"block-Jacobi FP64, {args.grouping} grouping, chunk {args.bj_chunk}",
+            "mixed_precond": (f"inner FGMRES to {args.inner_rtol:g} on the mixed operator, "
+                              f"block-Jacobi FP32, {args.grouping_mixed} grouping, chunk "
+                              f"{args.bj_chunk_mixed}"),
+            "learned_preconditioner": "not built yet",
         },
         "roofline": {"bound": "hbm", "achieved": gemv_bytes / gemv_ms / 1e6,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",

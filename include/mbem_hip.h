@@ -161,6 +161,50 @@ int mbem_bj_apply(const int64_t *wave_table, int64_t nwaves, const int64_t *chun
                   const int64_t *idx, const int64_t *inv_ptr, const double *inv, const double *r,
                   double *z, hipStream_t stream);
 
+/* ---- mixed precision (north-star pattern 1) ---------------------------- */
+
+/* out = lo(A) for an (m, n) row-major FP64 block: BF16 bits (bf16 != 0, round
+ * to nearest even) or FP32, leading dimension ldo.  Far-field storage of the
+ * FGMRES matvec (la/solver.py:178 operator, stored low). */
+int mbem_cast_lo(const double *A, int64_t lda, int64_t m, int64_t n, void *out, int64_t ldo,
+                 int bf16, hipStream_t stream);
+
+/* Near-field pattern over nb unknown blocks: block i is near source j when
+ * |pts_i - cen_j| < rad_j.  Two passes: with counts != NULL write the
+ * per-row counts; with col != NULL (and rowptr = exclusive prefix sum of the
+ * counts) write the ascending column lists. */
+int mbem_near_pattern(const double *pts, const double *cen, const double *rad, int64_t nb,
+                      int64_t *counts, const int64_t *rowptr, int64_t *col, hipStream_t stream);
+
+/* Near-field blocks: vals[b] (3x3 FP64) = A's block (row-block i, col[b]) for
+ * b in rowptr[i] .. rowptr[i+1]; the same entries of the low copy are zeroed,
+ * so A = lo + near exactly in the near blocks. */
+int mbem_near_extract(const double *A, int64_t lda, int64_t nbr, const int64_t *rowptr,
+                      const int64_t *col, int64_t nnzb, double *vals, void *lo, int64_t ldo,
+                      int bf16, hipStream_t stream);
+
+/* xf = (float) x: the FP32 copy of a Krylov vector that mbem_gemv_lo reads. */
+int mbem_vec_to_f32(const double *x, int64_t n, float *xf, hipStream_t stream);
+
+/* y = alpha F x + beta y, F stored BF16 / FP32 (16-byte aligned rows: ldf a
+ * multiple of 8 / 4), FP32 x (16-byte aligned), FP32 products and
+ * accumulation, FP64 y. */
+int mbem_gemv_lo(const void *F, int64_t ldf, int64_t m, int64_t n, const float *x, double *y,
+                 double alpha, double beta, int bf16, hipStream_t stream);
+
+/* y += alpha N x for a 3x3 block-CSR N in FP64 (the near field). */
+int mbem_bsr_gemv(int64_t nbr, const int64_t *rowptr, const int64_t *col, const double *vals,
+                  const double *x, double *y, double alpha, hipStream_t stream);
+
+/* FP32 block-Jacobi: chunk gather, strided-batched sgetrf + sgetri, apply. */
+int mbem_gather_block_f32(const double *A, int64_t lda, const int64_t *idx, int64_t m, float *M,
+                          hipStream_t stream);
+int mbem_invert_batched_f32(float *M, int64_t m, int64_t batch, int32_t *ipiv, int32_t *info,
+                            hipStream_t stream);
+int mbem_bj_apply_f32(const int64_t *wave_table, int64_t nwaves, const int64_t *chunk_ptr,
+                      const int64_t *idx, const int64_t *inv_ptr, const float *inv, const double *r,
+                      double *z, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

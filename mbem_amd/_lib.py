@@ -48,6 +48,15 @@ _SIGS = {
     "mbem_lu_factor": [_p, _i64, _p, _p, _p],
     "mbem_lu_solve": [_p, _i64, _p, _p, _i64, _p],
     "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
+    "mbem_cast_lo": [_p, _i64, _i64, _i64, _p, _i64, _int, _p],
+    "mbem_near_pattern": [_p, _p, _p, _i64, _p, _p, _p, _p],
+    "mbem_near_extract": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _i64, _int, _p],
+    "mbem_vec_to_f32": [_p, _i64, _p, _p],
+    "mbem_gemv_lo": [_p, _i64, _i64, _i64, _p, _p, _d, _d, _int, _p],
+    "mbem_bsr_gemv": [_i64, _p, _p, _p, _p, _p, _d, _p],
+    "mbem_gather_block_f32": [_p, _i64, _p, _i64, _p, _p],
+    "mbem_invert_batched_f32": [_p, _i64, _i64, _p, _p, _p],
+    "mbem_bj_apply_f32": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
     "mbem_src_fields": [],
     "mbem_multidot_work": [_i64, _i64],
 }

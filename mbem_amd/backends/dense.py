@@ -391,14 +391,18 @@ class AssembledDense:
               x0=None, precond_max_dense=None,
               precond_above_dense: str = defaults.PRECOND_RUNG_ABOVE_DENSE,
               precond_hodlr_max=None, precond_bj_chunk=None, recycle=None,
-              return_device: bool = False):
+              return_device: bool = False, precond_grouping: str = defaults.PRECOND_GROUPING):
         """{slot name: (n_dofs, 3)}; leaves ``self.report`` (SolveReport for
-        FGMRES, DenseSolveReport for LU)."""
+        FGMRES, DenseSolveReport for LU).  ``precond_grouping`` ("patch": a
+        Gauss-Seidel sweep over patches, the reference's default; "region":
+        one super-block per region, i.e. plain block-Jacobi over the cluster
+        chunks) is an addition to the reference's keywords."""
         if method == "lu":
             x = self._solve_lu()
         elif method == "fgmres":
             x = self._solve_fgmres(rtol, restart, maxiter, x0, precond_max_dense,
-                                   precond_above_dense, precond_bj_chunk, recycle)
+                                   precond_above_dense, precond_bj_chunk, recycle,
+                                   precond_grouping)
         else:
             raise ValueError(f"solve method {method!r} (expected 'fgmres' or 'lu')")
         self.x_dev = x
@@ -408,22 +412,23 @@ class AssembledDense:
         return {s.name: xh[s.offset:s.stop].reshape(-1, 3) for s in self.layout.slots}
 
     def preconditioner(self, max_dense=None, bj_chunk=None,
-                       above_dense=defaults.PRECOND_RUNG_ABOVE_DENSE):
+                       above_dense=defaults.PRECOND_RUNG_ABOVE_DENSE,
+                       grouping=defaults.PRECOND_GROUPING):
         from ..la.preconditioner import BlockGaussSeidel
-        caps = (max_dense, above_dense, bj_chunk)
+        caps = (max_dense, above_dense, bj_chunk, grouping)
         if self._precond is not None and self._precond_caps != caps:
             self._precond = None
         if self._precond is None:
             self._precond = BlockGaussSeidel(self, max_dense=max_dense, bj_chunk=bj_chunk,
-                                             above_dense=above_dense)
+                                             above_dense=above_dense, grouping=grouping)
             self._precond_caps = caps
         return self._precond
 
     def _solve_fgmres(self, rtol, restart, maxiter, x0, max_dense, above_dense, bj_chunk,
-                      recycle):
+                      recycle, grouping=defaults.PRECOND_GROUPING):
         from ..la.solver import fgmres
         t0 = time.perf_counter()
-        M = self.preconditioner(max_dense, bj_chunk, above_dense)
+        M = self.preconditioner(max_dense, bj_chunk, above_dense, grouping)
         self.timings["precond_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         if x0 is not None and not isinstance(x0, torch.Tensor):

@@ -1,0 +1,211 @@
+"""Mixed-precision solve of an assembled dense system (north-star pattern 1).
+
+Operator split A = F + N on the device:
+  N  near-field and self-element 3x3 blocks in FP64 block-CSR -- every
+     (collocation point, source triangle) pair closer than
+     ``k_near * (eps + h)`` of the source, where the closed-form log and
+     arctangent terms cancel;
+  F  everything else in BF16 (or FP32) storage, near entries zeroed,
+     multiplied with FP32 accumulation.
+
+Solve: the outer loop is FP64 FGMRES on the exact FP64 operator (Arnoldi,
+Givens and the true residual all FP64), so it is iterative refinement; its
+right preconditioner is an inner FGMRES on the mixed operator F + N,
+preconditioned by block Gauss-Seidel / block-Jacobi with FP32 chunk
+inverses and mixed-operator couplings, run to a loose tolerance.  The error
+against the FP64 solution is set by the outer rtol alone.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+import torch
+
+from .. import _lib, defaults
+from .._lib import call, ptr, stream
+from ..kernels import basis as kb
+from .solver import SolveReport, fgmres
+
+F64 = torch.float64
+I64 = torch.int64
+
+
+def near_inputs(assembled, k_near: float = 2.0):
+    """(pts, cen, rad) per unknown block: collocation point of row block i,
+    source centroid of column block j and its near radius k_near (eps + h)."""
+    layout = assembled.layout
+    n_blk = layout.n_unknowns // 3
+    pts = np.empty((n_blk, 3))
+    cen = np.empty((n_blk, 3))
+    rad = np.empty(n_blk)
+    for s in layout.slots:
+        p = s.patch
+        b0, b1 = s.offset // 3, s.stop // 3
+        pts[b0:b1] = assembled.collocation_points_of(p)
+        cen[b0:b1] = p.mesh.centroids()
+        rad[b0:b1] = k_near * (assembled.eps_for(p) + kb.element_sizes(p.mesh))
+    return pts, cen, rad
+
+
+def near_structure(assembled, k_near: float = 2.0):
+    """Block-CSR pattern (rowptr, col) of the near field over the system's
+    unknown blocks (row block i = collocation point of unknown triple i,
+    column block j = source element of unknown triple j): near when
+    |x_i - c_j| < k_near * (eps_j + h_j).  Counted and filled on the GPU
+    (mbem_near_pattern); returned as host arrays."""
+    pts, cen, rad = near_inputs(assembled, k_near)
+    dev = assembled.A_dev.device
+    nb = pts.shape[0]
+    P, C, Rd = (torch.as_tensor(a, dtype=F64, device=dev) for a in (pts, cen, rad))
+    counts = torch.empty(nb, dtype=I64, device=dev)
+    call("mbem_near_pattern", ptr(P), ptr(C), ptr(Rd), nb, ptr(counts), None, None, stream())
+    rowptr = torch.zeros(nb + 1, dtype=I64, device=dev)
+    rowptr[1:] = torch.cumsum(counts, 0)
+    nnz = int(rowptr[-1].item())
+    col = torch.empty(max(nnz, 1), dtype=I64, device=dev)
+    call("mbem_near_pattern", ptr(P), ptr(C), ptr(Rd), nb, None, ptr(rowptr), ptr(col), stream())
+    return rowptr.cpu().numpy(), col[:nnz].cpu().numpy()
+
+
+class MixedOperator:
+    """y = F x + N x on the device (see the module docstring)."""
+
+    def __init__(self, assembled, storage: str = "bf16", k_near: float = 2.0, pattern=None):
+        if storage not in ("bf16", "fp32"):
+            raise ValueError(f"storage {storage!r} (expected 'bf16' or 'fp32')")
+        _lib.require_gpu()
+        A = assembled.A_dev
+        n = A.shape[0]
+        dev = A.device
+        self.n = n
+        self.bf16 = storage == "bf16"
+        self.storage = storage
+        t0 = time.perf_counter()
+        align = 8 if self.bf16 else 4
+        self.ldf = ((n + align - 1) // align) * align
+        dt = torch.int16 if self.bf16 else torch.float32
+        self.F = torch.empty((n, self.ldf), dtype=dt, device=dev)
+        call("mbem_cast_lo", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf, int(self.bf16),
+             stream())
+        rowptr, col = pattern if pattern is not None else near_structure(assembled, k_near)
+        self.nbr = rowptr.size - 1
+        self.nnzb = int(rowptr[-1])
+        self.rowptr_h, self.col_h = rowptr, col
+        self.rowptr = torch.as_tensor(rowptr, dtype=I64, device=dev)
+        self.col = torch.as_tensor(col, dtype=I64, device=dev)
+        self.vals = torch.empty(9 * max(self.nnzb, 1), dtype=F64, device=dev)
+        call("mbem_near_extract", ptr(A), A.stride(0), self.nbr, ptr(self.rowptr), ptr(self.col),
+             self.nnzb, ptr(self.vals), ptr(self.F), self.ldf, int(self.bf16), stream())
+        self._lower = {}
+        self.xf = torch.empty(max(n, 4), dtype=torch.float32, device=dev)   # FP32 copy of x
+        torch.cuda.current_stream().synchronize()
+        self.build_s = time.perf_counter() - t0
+        self.far_bytes = n * n * (2 if self.bf16 else 4)
+        self.near_bytes = self.nnzb * (72 + 8)
+
+    def apply_into(self, v, out):
+        call("mbem_vec_to_f32", ptr(v), self.n, ptr(self.xf), stream())
+        call("mbem_gemv_lo", ptr(self.F), self.ldf, self.n, self.n, ptr(self.xf), ptr(out), 1.0,
+             0.0, int(self.bf16), stream())
+        call("mbem_bsr_gemv", self.nbr, ptr(self.rowptr), ptr(self.col), ptr(self.vals), ptr(v),
+             ptr(out), 1.0, stream())
+        return out
+
+    def matvec(self, v):
+        out = torch.empty_like(v)
+        return self.apply_into(v, out)
+
+    def _lower_bsr(self, r0, r1):
+        """Near blocks of rows [r0, r1) with columns < r0 (block-CSR)."""
+        key = (r0, r1)
+        if key not in self._lower:
+            b0, b1, c_lim = r0 // 3, r1 // 3, r0 // 3
+            rp = self.rowptr_h[b0:b1 + 1]
+            keep = self.col_h[rp[0]:rp[-1]] < c_lim
+            idx = np.arange(rp[0], rp[-1])[keep]
+            row_of = np.repeat(np.arange(b1 - b0), np.diff(rp))
+            counts = np.bincount(row_of[keep], minlength=b1 - b0).astype(np.int64)
+            rowptr = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
+            dev = self.vals.device
+            gather = torch.as_tensor(idx, dtype=I64, device=dev)
+            vals = self.vals.view(-1, 9)[gather].reshape(-1).contiguous() if idx.size else \
+                torch.zeros(9, dtype=F64, device=dev)
+            self._lower[key] = (b1 - b0, torch.as_tensor(rowptr, dtype=I64, device=dev),
+                                torch.as_tensor(self.col_h[idx], dtype=I64, device=dev), vals)
+        return self._lower[key]
+
+    def lower_into(self, r0, r1, z, out):
+        """out -= (F + N)[r0:r1, :r0] z[:r0]: the Gauss-Seidel lower coupling."""
+        base = self.F[r0:r1]
+        call("mbem_vec_to_f32", ptr(z), r0, ptr(self.xf), stream())
+        call("mbem_gemv_lo", ptr(base), self.ldf, r1 - r0, r0, ptr(self.xf), ptr(out), -1.0, 1.0,
+             int(self.bf16), stream())
+        nbr, rowptr, col, vals = self._lower_bsr(r0, r1)
+        call("mbem_bsr_gemv", nbr, ptr(rowptr), ptr(col), ptr(vals), ptr(z), ptr(out), -1.0,
+             stream())
+
+
+class _InnerSolve:
+    """Right preconditioner of the outer loop: an inner FGMRES on the mixed
+    operator to a loose tolerance."""
+
+    def __init__(self, op, M, rtol, maxiter):
+        self.op, self.M, self.rtol, self.maxiter = op, M, rtol, maxiter
+        self.iterations = []
+
+    def apply_into(self, r, z):
+        x, rep = fgmres(self.op, r, M=self.M, rtol=self.rtol, restart=self.maxiter,
+                        maxiter=self.maxiter)
+        self.iterations.append(rep.iterations)
+        z.copy_(x)
+        return z
+
+
+@dataclass
+class MixedReport:
+    outer: SolveReport
+    inner_iterations: list = field(default_factory=list)
+    operator_build_s: float = 0.0
+    precond_build_s: float = 0.0
+    solve_s: float = 0.0
+    storage: str = "bf16"
+    near_blocks: int = 0
+    far_bytes: int = 0
+
+    @property
+    def converged(self):
+        return self.outer.converged
+
+    @property
+    def iterations(self):
+        return self.outer.iterations
+
+    def __str__(self):
+        return (f"mixed ({self.storage} far field, FP64 near + outer): outer {self.outer}, "
+                f"inner iterations {self.inner_iterations}")
+
+
+def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
+                inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
+                outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=3000,
+                precond_grouping: str = "region", pattern=None):
+    """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport)."""
+    from .preconditioner import BlockGaussSeidel
+    t0 = time.perf_counter()
+    op = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pattern)
+    t1 = time.perf_counter()
+    M = BlockGaussSeidel(assembled, max_dense=precond_max_dense, bj_chunk=precond_bj_chunk,
+                         grouping=precond_grouping, inverse_dtype="f32", coupling=op)
+    torch.cuda.current_stream().synchronize()
+    t2 = time.perf_counter()
+    inner = _InnerSolve(op, M, inner_rtol, inner_maxiter)
+    x, outer = fgmres(assembled, assembled.b_dev, M=inner, rtol=rtol, restart=outer_restart,
+                      maxiter=outer_maxiter)
+    torch.cuda.current_stream().synchronize()
+    t3 = time.perf_counter()
+    rep = MixedReport(outer=outer, inner_iterations=inner.iterations, operator_build_s=t1 - t0,
+                      precond_build_s=t2 - t1, solve_s=t3 - t2, storage=storage,
+                      near_blocks=op.nnzb, far_bytes=op.far_bytes)
+    return x, rep

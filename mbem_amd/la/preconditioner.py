@@ -142,7 +142,7 @@ class BlockGaussSeidel:
     def __init__(self, assembled, max_dense=None, bj_chunk=None,
                  grouping: str = defaults.PRECOND_GROUPING,
                  above_dense: str = defaults.PRECOND_RUNG_ABOVE_DENSE, verbose: bool = False,
-                 hodlr_tol=None, hodlr_max=None):
+                 hodlr_tol=None, hodlr_max=None, inverse_dtype: str = "f64", coupling=None):
         _lib.require_gpu()
         if max_dense is None:
             max_dense = dense_rung_max_dof()
@@ -153,6 +153,10 @@ class BlockGaussSeidel:
                                       f"'block_jacobi' is in this build")
         if grouping == "region" and getattr(assembled, "deflate", False):
             grouping = "patch"
+        if inverse_dtype not in ("f64", "f32"):
+            raise ValueError(f"inverse_dtype {inverse_dtype!r} (expected 'f64' or 'f32')")
+        self.inverse_dtype = inverse_dtype
+        self.coupling = coupling
         self.max_dense = int(max_dense)
         self.bj_chunk = int(bj_chunk)
         self.above_dense = above_dense
@@ -192,13 +196,14 @@ class BlockGaussSeidel:
         self.chunk_ptr = torch.as_tensor(cptr, dtype=I64, device=dev)
         self.inv_ptr = torch.as_tensor(iptr, dtype=I64, device=dev)
         self.idx = torch.as_tensor(np.concatenate([c[1] for c in chunks]), dtype=I64, device=dev)
-        self.inv = torch.empty(int(iptr[-1]), dtype=F64, device=dev)
+        f32 = inverse_dtype == "f32"
+        self.inv = torch.empty(int(iptr[-1]), dtype=torch.float32 if f32 else F64, device=dev)
         R = _lib.BJ_ROWS_PER_WAVE
         tables = {k: [] for k in range(len(self.sbs))}
         for ci, (k, dofs) in enumerate(chunks):
             m = dofs.size
             M = self.inv[int(iptr[ci]):int(iptr[ci + 1])]
-            call("mbem_gather_block", ptr(A), A.stride(0),
+            call("mbem_gather_block_f32" if f32 else "mbem_gather_block", ptr(A), A.stride(0),
                  ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), m, ptr(M), stream())
             tables[k] += [(ci, r0) for r0 in range(0, m, R)]
         # invert runs of equal-size chunks (contiguous in ``inv``) as one batch
@@ -210,8 +215,8 @@ class BlockGaussSeidel:
             m, nb = int(sizes[ci]), cj - ci + 1
             ipiv = torch.empty(nb * m, dtype=torch.int32, device=dev)
             info = torch.zeros(nb, dtype=torch.int32, device=dev)
-            call("mbem_invert_batched", ptr(self.inv[int(iptr[ci]):]), m, nb, ptr(ipiv),
-                 ptr(info), stream())
+            call("mbem_invert_batched_f32" if f32 else "mbem_invert_batched",
+                 ptr(self.inv[int(iptr[ci]):]), m, nb, ptr(ipiv), ptr(info), stream())
             bad = info.cpu().numpy()
             if np.any(bad != 0):
                 k = int(np.flatnonzero(bad)[0])
@@ -225,7 +230,7 @@ class BlockGaussSeidel:
         torch.cuda.current_stream().synchronize()
         self.build_s = time.perf_counter() - t0
         self.n_chunks = len(chunks)
-        self.inverse_bytes = int(iptr[-1]) * 8
+        self.inverse_bytes = int(iptr[-1]) * (4 if f32 else 8)
         self._rr = torch.empty(n, dtype=F64, device=dev)
 
     def apply_into(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
@@ -235,10 +240,14 @@ class BlockGaussSeidel:
             seg = slice(sb.start, sb.stop)
             rr[seg].copy_(r[seg])
             if sb.start > 0:
-                call("mbem_gemv", ptr(A[sb.start:sb.stop]), A.stride(0), sb.size, sb.start,
-                     ptr(z), ptr(rr[seg]), -1.0, 1.0, stream())
-            call("mbem_bj_apply", ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr),
-                 ptr(self.idx), ptr(self.inv_ptr), ptr(self.inv), ptr(rr), ptr(z), stream())
+                if self.coupling is not None:
+                    self.coupling.lower_into(sb.start, sb.stop, z, rr[seg])
+                else:
+                    call("mbem_gemv", ptr(A[sb.start:sb.stop]), A.stride(0), sb.size, sb.start,
+                         ptr(z), ptr(rr[seg]), -1.0, 1.0, stream())
+            call("mbem_bj_apply_f32" if self.inverse_dtype == "f32" else "mbem_bj_apply",
+                 ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr), ptr(self.idx),
+                 ptr(self.inv_ptr), ptr(self.inv), ptr(rr), ptr(z), stream())
         return z
 
     def __call__(self, r: torch.Tensor) -> torch.Tensor:
@@ -251,6 +260,7 @@ class BlockGaussSeidel:
             "grouping": self.grouping, "above_dense": self.above_dense,
             "max_dense": self.max_dense, "bj_chunk": self.bj_chunk, "build_s": self.build_s,
             "chunks": self.n_chunks, "inverse_bytes": self.inverse_bytes,
+            "inverse_dtype": self.inverse_dtype,
             "super_blocks": [{"slots": [s.name for s in sb.slots], "size": sb.size,
                               "rung": sb.rung, "build_s": sb.build_s, **sb.detail}
                              for sb in self.sbs],
