@@ -205,6 +205,84 @@ int mbem_bj_apply_f32(const int64_t *wave_table, int64_t nwaves, const int64_t *
                       const int64_t *idx, const int64_t *inv_ptr, const float *inv, const double *r,
                       double *z, hipStream_t stream);
 
+/* ---- Batched Laplace-state solves (rheology.py:479-524, dense.py:650-687) ----
+ * The reference solves its Laplace states one after another: a re-assembly
+ * or recombination of the bases at each state's moduli (rebuild_for_materials)
+ * and a solve per state (runner.py:357-391).  These entries serve one solve
+ * of every (state, right-hand side) pair at once: with A(s) = (A0 + g(s) A1)
+ * D(s) and y = D x, column c of a row-major n x K block is system
+ * B(g_c) y_c = b_c.  None of them replaces a reference function one-for-one;
+ * together they stand in for that state loop. */
+
+/* Tile configurations of the FP64 MFMA GEMM (the autotuner's candidates):
+ * the count, and {BM, BN, BK, waves along M, waves along N} of one. */
+int mbem_mgemm_configs(void);
+int mbem_mgemm_config(int cfg, int32_t *out5);
+
+/* Y = alpha (A0 X + A1 X diag(g)) + beta Y: A0, A1 row-major m x kd (even
+ * lda, 16-byte aligned), X kd x ncols (even ldx, aligned), g (ncols).
+ * FP64 matrix cores, tile configuration cfg. */
+int mbem_gemm_g(const double *A0, const double *A1, int64_t lda, int64_t m, int64_t kd,
+                const double *X, int64_t ldx, int64_t ncols, const double *g, double *Y,
+                int64_t ldy, double alpha, double beta, int cfg, hipStream_t stream);
+
+/* Grouped GEMM over a device work table of nwork items (mbem_work_item_bytes
+ * each: chunk inverse offset and leading dimension, chunk size, block offset,
+ * column count, tile row, tile column): Y[rows of chunk, cols] = Inv X[...].
+ * The batched block-Jacobi apply (preconditioner.py:601-630 for many
+ * columns). */
+int mbem_work_item_bytes(void);
+int mbem_gemm_grouped(const double *inv, const double *X, int64_t ldx, double *Y, int64_t ldy,
+                      const void *work, int64_t nwork, int cfg, hipStream_t stream);
+
+/* Column-wise multi-dot of cnt (<= 8) blocks V_i (stride bstride) with W:
+ * partial sums over nslots row slices (a multiple of 4), then reduced into
+ * out[c * ldo + o0 + i] (set, or added when accumulate); copy (cnt x K per
+ * column) optional.  The Arnoldi projections of K runs (solver.py:115-245). */
+int mbem_bdot(const double *V, int64_t bstride, int64_t cnt, const double *W, int64_t n, int64_t K,
+              double *partial, int64_t nslots, hipStream_t stream);
+int mbem_breduce(const double *partial, int64_t nslots, int64_t cnt, int64_t K, double *out,
+                 int64_t ldo, int64_t o0, int accumulate, double *copy, hipStream_t stream);
+
+/* Y += sign * sum_{i < cnt} coef[c * ldc + i] B_i (cnt read from cnt_dev when
+ * non-null); Y = X diag(scale); Y = X; y[:] = v. */
+int mbem_baxpy(const double *B, int64_t bstride, int64_t cnt, const int *cnt_dev,
+               const double *coef, int64_t ldc, double sign, double *Y, int64_t n, int64_t K,
+               hipStream_t stream);
+int mbem_bscale(const double *X, const double *scale, double *Y, int64_t n, int64_t K,
+                hipStream_t stream);
+int mbem_bcopy(const double *X, double *Y, int64_t nk, hipStream_t stream);
+int mbem_fill(double *y, int64_t n, double v, hipStream_t stream);
+
+/* FGMRES bookkeeping on the device, one thread per run: cycle start
+ * (beta, 1/beta, gvec), Arnoldi step j (H column j = hp1 + hp2, the two
+ * CGS passes, K x (m+1) each; H[j+1, j]; Givens rotation j; gvec; residual
+ * estimate; breakdown zeroes the next basis scale), and
+ * y = R^{-1} gvec over the steps done (solver.py:115-245, _solve_upper
+ * solver.py:503-510). */
+int mbem_fg_start(int64_t K, int64_t m, const double *sumsq, double *beta, double *scale,
+                  double *gvec, int *jdone, hipStream_t stream);
+int mbem_fg_givens(int64_t K, int64_t m, int64_t j, const double *hp1, const double *hp2, double *H,
+                   double *cs, double *sn, double *gvec, const double *sumsq, double *scale,
+                   double *res, int *jdone, hipStream_t stream);
+int mbem_fg_solve(int64_t K, int64_t m, const double *H, const double *gvec, const int *jdone,
+                  double *y, hipStream_t stream);
+
+/* out = A0[idx, idx] + g A1[idx, idx] (m x m): a chunk of B(g); out = A[idx, idx]
+ * for a full permutation into cluster order. */
+int mbem_gather_comb(const double *A0, const double *A1, int64_t lda, const int64_t *idx, int64_t m,
+                     double g, double *out, hipStream_t stream);
+int mbem_permute(const double *A, int64_t lda, const int64_t *idx, int64_t n, double *out,
+                 int64_t ldo, hipStream_t stream);
+
+/* HIP graphs: capture the launches made on a (non-null) stream, instantiate,
+ * replay, destroy; and an async copy that a graph can hold. */
+int mbem_graph_begin(hipStream_t stream);
+int mbem_graph_end(hipStream_t stream, void **exec_out);
+int mbem_graph_launch(void *exec, hipStream_t stream);
+int mbem_graph_destroy(void *exec);
+int mbem_memcpy_async(void *dst, const void *src, int64_t bytes, hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

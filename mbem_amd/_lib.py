@@ -59,6 +59,28 @@ _SIGS = {
     "mbem_bj_apply_f32": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
     "mbem_src_fields": [],
     "mbem_multidot_work": [_i64, _i64],
+    # batched Laplace-state solves
+    "mbem_mgemm_configs": [],
+    "mbem_mgemm_config": [_int, _p],
+    "mbem_gemm_g": [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _int, _p],
+    "mbem_work_item_bytes": [],
+    "mbem_gemm_grouped": [_p, _p, _i64, _p, _i64, _p, _i64, _int, _p],
+    "mbem_bdot": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p],
+    "mbem_breduce": [_p, _i64, _i64, _i64, _p, _i64, _i64, _int, _p, _p],
+    "mbem_baxpy": [_p, _i64, _i64, _p, _p, _i64, _d, _p, _i64, _i64, _p],
+    "mbem_bscale": [_p, _p, _p, _i64, _i64, _p],
+    "mbem_bcopy": [_p, _p, _i64, _p],
+    "mbem_fill": [_p, _i64, _d, _p],
+    "mbem_fg_start": [_i64, _i64, _p, _p, _p, _p, _p, _p],
+    "mbem_fg_givens": [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
+    "mbem_fg_solve": [_i64, _i64, _p, _p, _p, _p, _p],
+    "mbem_gather_comb": [_p, _p, _i64, _p, _i64, _d, _p, _p],
+    "mbem_permute": [_p, _i64, _p, _i64, _p, _i64, _p],
+    "mbem_graph_begin": [_p],
+    "mbem_graph_end": [_p, _p],
+    "mbem_graph_launch": [_p, _p],
+    "mbem_graph_destroy": [_p],
+    "mbem_memcpy_async": [_p, _p, _i64, _p],
 }
 _RESTYPE = {"mbem_multidot_work": _i64}
 

@@ -167,7 +167,7 @@ class AssemblyReport:
 class AssembledDense:
     def __init__(self, system: BlockSystem, eps, mode: str = "direct", jump: str = "calibrated",
                  deflate: bool = False, lu=None, calibrate_all_rows: bool = False,
-                 fault_drag=None, _materials=None):
+                 fault_drag=None, _materials=None, _coeffs=None):
         if jump not in ("half", "calibrated"):
             raise ValueError(jump)
         if fault_drag:
@@ -186,6 +186,10 @@ class AssembledDense:
         self.deflate = deflate
         self.lu = lu
         self.calibrate_all_rows = calibrate_all_rows
+        # (kernel, region) -> basis coefficients; None = from the region's
+        # moduli.  The Laplace-state split assembles the g-free and g parts
+        # of the operator through it.
+        self._coeff_fn = _coeffs
         self.materials = dict(_materials) if _materials is not None else \
             {r.name: r.material for r in system.model.regions}
         self.fault_drag = {}
@@ -238,6 +242,8 @@ class AssembledDense:
 
     # -- assembly -----------------------------------------------------------
     def _coeffs(self, kernel, region):
+        if self._coeff_fn is not None:
+            return np.asarray(self._coeff_fn(kernel, region), dtype=float)
         m = self.materials[region.name]
         return kb.t_coeffs(m.mu, m.lam) if kernel == KERNEL_T else kb.u_coeffs(m.mu, m.lam)
 
