@@ -46,6 +46,16 @@ def _measured_traffic_ratio():
     return None, None
 
 
+def _executed_flops():
+    """{kernel: executed FP64 flops per pair} from the committed PMC summary."""
+    import re
+    path = os.path.join(ROOT, "profiles", "fp64_executed_flops.md")
+    if not os.path.exists(path):
+        return {}
+    return {m.group(1): float(m.group(2)) for m in
+            re.finditer(r"executed_flops_per_pair (\w+) = ([0-9.]+)", open(path).read())}
+
+
 def _events_time(fn, reps):
     import torch
     s = torch.cuda.Event(enable_timing=True)
@@ -150,6 +160,262 @@ def kernel_probe(n: int, reps: int):
                       "copy_bytes_each_way": n * n * 8, "reps": reps}), flush=True)
 
 
+def bench_laplace(args):
+    """Batched Laplace-state solves: one relaxing Maxwell region (eta = 1e19
+    Pa s, mu = 30 GPa, tau = eta/mu), 32 s values from s_grid(0.01 tau,
+    2e7 tau), 8 random unit slip fields (seed 5) on the fault, every
+    (state, slip) pair solved in one lockstep batch (K = 256 columns) to
+    rtol 1e-10.  Prints one JSON line of its own."""
+    import torch
+    from mbem_amd import laplace, rheology as rh
+    from mbem_amd.backends.dense import AssembledDense
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.kernels import basis as kb
+    from mbem_amd.model import generate_system
+    from mbem_amd.viscoelastic import LaplaceStates
+
+    n_cells = args.laplace_cells
+    model, eps, info = box_fault_model(n_cells, fault_cells=tuple(args.laplace_fault_cells))
+    region = model.regions[0]
+    h = float(np.mean([kb.element_sizes(p.mesh).mean() for p in region.patches]))
+    eps = {p.name: 0.5 * h for p in region.patches}
+    eps["fault"] = 0.5 * float(kb.element_sizes(region.faults[0].mesh).mean())
+    system = generate_system(model)
+    n = system.layout.n_unknowns
+    tau = 1.0e19 / 3.0e10
+    grid = laplace.s_grid(0.01 * tau, 2e7 * tau)
+    states = rh.laplace_states({"crust": (30.0, 30.0)}, {"crust": rh.Maxwell(tau)}, grid)[1:]
+    rng = np.random.default_rng(5)
+    nf = region.faults[0].mesh.n_triangles
+    slips = []
+    for _ in range(args.laplace_rhs):
+        v = rng.normal(size=(nf, 3))
+        slips.append({"fault": v / np.linalg.norm(v, axis=1, keepdims=True)})
+    print(f"[laplace] N={info['box_triangles']} n={n} fault={nf} states={len(states)} "
+          f"rhs={len(slips)}", flush=True)
+    times = []
+    x = rep = ls = None
+    for k in range(args.warmup + args.steps):
+        ls = None
+        torch.cuda.empty_cache()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ls = LaplaceStates(system, eps, {"crust": rh.Maxwell(tau)}, bj_chunk=args.bj_chunk)
+        x, rep = ls.solve(states, slips, rtol=args.laplace_rtol, restart=args.laplace_restart,
+                          n_anchors=args.laplace_anchors)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        tag = "warmup" if k < args.warmup else "step"
+        print(f"[laplace] {tag} {k}: {dt:.3f} s ({rep})", flush=True)
+        if k >= args.warmup:
+            times.append(dt)
+    # roofline of the batched operator GEMM, timed live with events on its stream
+    from mbem_amd.la.batched import GOperator
+    K = len(states) * len(slips)
+    K += K & 1
+    g = torch.full((K,), 1.5, dtype=torch.float64, device="cuda")
+    op = GOperator(ls.A[0], ls.A[1], g)
+    op.cfg = rep.gemm_cfg["gemm_g"]
+    X = torch.randn(n, K, dtype=torch.float64, device="cuda")
+    Y = torch.empty_like(X)
+    gemm_ms = _events_time(lambda: op.apply(X, Y), 5)
+    flops = op.flops
+    # sequential comparison on sampled states: the per-state path of the
+    # reference (re-assembly at the state's moduli, then a solve per slip)
+    seq = []
+    errs = []
+    pick = sorted({0, len(states) // 2, len(states) - 1})[:args.laplace_seq_states]
+    b0, b1 = ls.rhs_parts(slips)
+    for i in pick:
+        st = states[i]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        asm = AssembledDense(system, eps, _materials=dict(st.materials))
+        mu, lam, gval = ls.state_moduli(st.s)
+        M = asm.preconditioner(0, args.bj_chunk, grouping="region")
+        from mbem_amd.la.solver import fgmres
+        xs = []
+        for r in range(len(slips)):
+            b = (b0[:, r] + gval * b1[:, r]).contiguous()
+            xr, srep = fgmres(asm, b, M=M, rtol=args.laplace_rtol, restart=200, maxiter=600)
+            xs.append(xr)
+        torch.cuda.synchronize()
+        seq.append(time.perf_counter() - t0)
+        for r in range(len(slips)):
+            ref = xs[r].cpu().numpy()
+            errs.append(float(np.linalg.norm(x[i, r] - ref) / np.linalg.norm(ref)))
+        del asm, M
+        torch.cuda.empty_cache()
+        print(f"[laplace] sequential state {st.label}: {seq[-1]:.3f} s, "
+              f"max rel diff vs batched {max(errs[-len(slips):]):.2e}", flush=True)
+    seq_est = float(np.mean(seq)) * len(states)
+    value = float(np.mean(times))
+    line = {
+        "metric": "batched Laplace-state solve time (32 s x 8 RHS, one Maxwell region)",
+        "value": value, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * value, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic box-with-dipping-fault mesh generated in bench.py",
+        "config": {
+            "workload": f"box 40x40x20 km, N={info['box_triangles']} P0 triangles (n={n}), "
+                        f"fault {nf} triangles, Maxwell tau = 1e19 Pa s / 30 GPa, "
+                        f"{len(states)} s values x {len(slips)} random slips",
+            "rtol": args.laplace_rtol, "restart": args.laplace_restart,
+            "anchors": args.laplace_anchors, "bj_chunk": args.bj_chunk,
+            "assembly_s": rep.assembly_s, "precond_s": rep.precond_s, "solve_s": rep.solve_s,
+            "lockstep_steps": rep.batched.steps, "cycles": rep.batched.cycles,
+            "max_relres": float(rep.batched.relres.max()),
+            "gemm_cfg": rep.gemm_cfg,
+            "sequential_states_timed": len(pick),
+            "sequential_per_state_s": [float(v) for v in seq],
+            "sequential_estimate_s": seq_est,
+            "speedup_vs_sequential": seq_est / value,
+            "max_rel_diff_vs_sequential": max(errs) if errs else None,
+        },
+        "roofline": {"bound": "mfma", "achieved": flops / gemm_ms / 1e9,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": flops / gemm_ms / 1e9 / FP64_PEAK_TFLOPS, "traffic": None,
+                     "kernel": "k_mgemm (A0 X + A1 X diag(g), FP64 MFMA)",
+                     "algorithmic_flops": flops, "avg_ms": gemm_ms},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def readout_workload(n_points, seed_pts=11):
+    """Sources: the N = 20,000 box + 676-triangle fault mesh, per-element
+    eps = 0.5 h, random unit densities (seed 3).  Points: half uniform in the
+    box interior away from the faces (0.1 h), half within 3 eps of the fault
+    plane (uniform over the fault, normal offset U(-3 eps, 3 eps))."""
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.kernels import basis as kb
+    model, _, info = box_fault_model(50, fault_cells=(13, 26))
+    region = model.regions[0]
+    patches = list(region.patches) + list(region.faults)
+    tv, nrm, eps = [], [], []
+    for p in patches:
+        a, b = kb._source_arrays(p.mesh)
+        tv.append(a)
+        nrm.append(b)
+        eps.append(0.5 * kb.element_sizes(p.mesh))
+    tv, nrm, eps = np.concatenate(tv), np.concatenate(nrm), np.concatenate(eps)
+    dens = np.random.default_rng(3).normal(size=(tv.shape[0], 3))
+    dens /= np.linalg.norm(dens, axis=1, keepdims=True)
+    rng = np.random.default_rng(seed_pts)
+    h = 40.0 / 50
+    nb = n_points // 2
+    lo = np.array([-20.0, -20.0, -20.0]) + 0.1 * h
+    hi = np.array([20.0, 20.0, 0.0]) - 0.1 * h
+    box_pts = lo + (hi - lo) * rng.random((nb, 3))
+    fault = region.faults[0]
+    fv = fault.mesh.vertices[fault.mesh.triangles]
+    area = 0.5 * np.linalg.norm(np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0]), axis=1)
+    nf = n_points - nb
+    tri = rng.choice(fv.shape[0], size=nf, p=area / area.sum())
+    u, v = rng.random(nf), rng.random(nf)
+    flip = u + v > 1
+    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
+    base = fv[tri, 0] + u[:, None] * (fv[tri, 1] - fv[tri, 0]) + v[:, None] * (fv[tri, 2] - fv[tri, 0])
+    fn_ = np.cross(fv[0, 1] - fv[0, 0], fv[0, 2] - fv[0, 0])
+    fn_ /= np.linalg.norm(fn_)
+    eps_f = 0.5 * float(kb.element_sizes(fault.mesh).mean())
+    fault_pts = base + rng.uniform(-3 * eps_f, 3 * eps_f, size=(nf, 1)) * fn_[None, :]
+    pts = np.concatenate([box_pts, fault_pts])
+    return pts, tv, nrm, eps, dens, info
+
+
+def bench_readout(args):
+    """Stress-readout throughput: dd_stress_contract - eigenstress_contract,
+    FP64, full sum, 10^6 points x 20,676 sources.  Prints one JSON line."""
+    import torch
+    from mbem_amd.kernels import device as dv
+    mu, lam = 30.0, 30.0
+    pts, tv, nrm, eps, dens, info = readout_workload(args.readout_points)
+    ns = tv.shape[0]
+    print(f"[readout] points={pts.shape[0]} sources={ns}", flush=True)
+    dev = torch.device("cuda")
+    src = dv.Sources(torch.from_numpy(tv).to(dev), torch.from_numpy(nrm).to(dev),
+                     torch.from_numpy(eps).to(dev))
+    X = torch.from_numpy(pts).to(dev)
+    D = torch.from_numpy(dens).to(dev)
+    out = torch.empty((X.shape[0], 3, 3), dtype=torch.float64, device=dev)
+    st = torch.cuda.current_stream()
+
+    chunk = 131072          # points per launch (a few seconds of work each)
+
+    def launches(kind, scale, beta):
+        for p0 in range(0, X.shape[0], chunk):
+            p1 = min(p0 + chunk, X.shape[0])
+            dv.stress(kind, X[p0:p1], src, D, mu, lam, out[p0:p1], scale=scale, beta=beta)
+
+    # small warm launch, then the timed launches with events on the stream
+    dv.stress("dd", X[:1024], src, D, mu, lam, out[:1024])
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    times_dd, times_eig = [], []
+    for _ in range(max(1, args.steps)):
+        ev[0].record(st)
+        launches("dd", 1.0, 0.0)
+        ev[1].record(st)
+        launches("eigen", -1.0, 1.0)
+        ev[2].record(st)
+        ev[2].synchronize()
+        times_dd.append(ev[0].elapsed_time(ev[1]) / 1e3)
+        times_eig.append(ev[1].elapsed_time(ev[2]) / 1e3)
+        print(f"[readout] dd {times_dd[-1]:.3f} s, eigen {times_eig[-1]:.3f} s", flush=True)
+    t_dd, t_eig = float(np.mean(times_dd)), float(np.mean(times_eig))
+    pairs = float(pts.shape[0]) * ns
+    fl = json.load(open(os.path.join(ROOT, "profiles", "flops_per_pair.json")))
+    f_dd, f_eig = fl["dd_stress_pair"]["flops"], fl["eigenstress_pair"]["flops"]
+    ex = _executed_flops()
+    total = t_dd + t_eig
+    # parity and CPU baseline on a bounded sample of the same points
+    cpu = None
+    err = None
+    if not args.no_cpu_baseline:
+        from oracle import tri_kernels as ok
+        ncpu = args.readout_cpu_points
+        sel = np.random.default_rng(7).choice(pts.shape[0], size=ncpu, replace=False)
+        t0 = time.perf_counter()
+        ref = ok.dd_stress_contract(pts[sel], tv, nrm, eps, dens, mu, lam) \
+            - ok.eigenstress_contract(pts[sel], tv, nrm, eps, dens, mu, lam)
+        t_cpu = time.perf_counter() - t0
+        got = out.cpu().numpy()[sel]
+        err = float(np.abs(got - ref).max() / np.abs(ref).max())
+        cores = len(os.sched_getaffinity(0))
+        cpu = {"value": t_cpu * pts.shape[0] / ncpu, "unit": "s", "cores": cores, "kind": "port",
+               "sample": f"oracle C restatement (OpenMP) on {ncpu} of the {pts.shape[0]} points, "
+                         f"all {ns} sources, extrapolated linearly in points "
+                         f"({t_cpu:.1f} s measured)"}
+        print(f"[readout] oracle on {ncpu} points: {t_cpu:.1f} s, max rel err {err:.2e}",
+              flush=True)
+    line = {
+        "metric": "stress readout time (dd - eigen, FP64, full sum)",
+        "value": total, "unit": "s", "n_gpus": 1, "steps": args.steps, "warmup": 1,
+        "ms_per_step": 1e3 * total, "higher_is_better": False, "scaling": "strong",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic points and densities generated in bench.py",
+        "config": {"workload": f"{pts.shape[0]} observation points x {ns} source triangles "
+                               f"(N=20,000 box + 676 fault), per-element eps = 0.5 h",
+                   "pairs": pairs, "dd_s": t_dd, "eigen_s": t_eig,
+                   "pairs_per_s": pairs / total,
+                   "flops_per_pair_as_written": {"dd": f_dd, "eigen": f_eig},
+                   "algorithmic_tflops_as_written_dd": pairs * f_dd / t_dd / 1e12,
+                   "executed_flops_per_pair": {"dd": ex.get("dd"), "eigen": ex.get("eigen")},
+                   "max_rel_err_vs_oracle_sample": err},
+        "roofline": {"bound": "fp64-valu",
+                     "achieved": pairs * ex["dd"] / t_dd / 1e12 if "dd" in ex else None,
+                     "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                     "frac": pairs * ex["dd"] / t_dd / 1e12 / FP64_PEAK_TFLOPS if "dd" in ex else None,
+                     "traffic": None,
+                     "kernel": "k_contract<3> (dd stress; executed FP64 flops per pair from the "
+                               "PMC summary fp64_executed_flops.md)",
+                     "avg_ms": 1e3 * t_dd},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +433,18 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=3000)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
+    ap.add_argument("--workload", choices=("solve", "laplace", "readout"), default="solve",
+                    help="solve: the N=20k dense solve (the headline metric); laplace: the "
+                         "batched Laplace-state solves; readout: stress readout throughput")
+    ap.add_argument("--readout-points", type=int, default=1_000_000)
+    ap.add_argument("--readout-cpu-points", type=int, default=200)
+    ap.add_argument("--laplace-cells", type=int, default=36)
+    ap.add_argument("--laplace-fault-cells", type=int, nargs=2, default=(9, 18))
+    ap.add_argument("--laplace-rhs", type=int, default=8)
+    ap.add_argument("--laplace-rtol", type=float, default=1e-10)
+    ap.add_argument("--laplace-restart", type=int, default=30)
+    ap.add_argument("--laplace-anchors", type=int, default=4)
+    ap.add_argument("--laplace-seq-states", type=int, default=2)
     ap.add_argument("--probe-traffic", type=int, default=0, metavar="N",
                     help="only run the matvec / copy traffic probe on an N x N matrix")
     args = ap.parse_args()
@@ -177,6 +455,12 @@ def main():
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
               f"(the dense path's sharding is described, not run)", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "laplace":
+        bench_laplace(args)
+        return
+    if args.workload == "readout":
+        bench_readout(args)
+        return
     import torch
     from mbem_amd.backends.dense import AssembledDense
     from mbem_amd.cases import box_fault_model
