@@ -283,6 +283,65 @@ int mbem_graph_launch(void *exec, hipStream_t stream);
 int mbem_graph_destroy(void *exec);
 int mbem_memcpy_async(void *dst, const void *src, int64_t bytes, hipStream_t stream);
 
+/* ---- Learned FGMRES preconditioner (no counterpart in the reference) ----
+ * A point-cloud network over the boundary elements that adds a correction to
+ * the block-Jacobi output inside FGMRES (the reference's preconditioner,
+ * preconditioner.py:314-543, stays the fallback and the comparison).  FP32
+ * training kernels (forward, backward, Adam) and BF16 matrix-core inference. */
+
+/* k (8 or 16) nearest neighbours of each of P points (FP32 xyz), self
+ * excluded, ties by index: nbr (P x k). */
+int mbem_knn(const float *pts, int64_t P, int k, int32_t *nbr, hipStream_t stream);
+
+/* FP32 GEMM with free strides: C = alpha A B + beta C (+ bias[j]) (relu),
+ * A(i,k) = A[i sa0 + k sa1], B(k,j) = B[k sb0 + j sb1], C row-major (ldc).
+ * splits > 1: split-K through work (splits x M x N floats; no bias/relu). */
+int mbem_sgemm(const float *A, int64_t sa0, int64_t sa1, const float *B, int64_t sb0, int64_t sb1,
+               float *C, int64_t ldc, int64_t M, int64_t N, int64_t K, float alpha, float beta,
+               const float *bias, int relu, int64_t splits, float *work, hipStream_t stream);
+
+/* out[c] = beta out[c] + sum_i X[i, c] (X: P x C; work: nslices x C floats). */
+int mbem_colsum(const float *X, int64_t P, int64_t C, float *out, float *work, int64_t nslices,
+                float beta, hipStream_t stream);
+
+/* kNN mean m_i = (1/k) sum_q a[nbr[i,q]] and its backward through the
+ * reverse graph (CSR rptr/ridx of in-neighbours). */
+int mbem_knn_mean(const float *a, int64_t P, int64_t C, const int32_t *nbr, int k, float *m,
+                  hipStream_t stream);
+int mbem_knn_mean_bwd(const float *dm, int64_t P, int64_t C, const int32_t *rptr,
+                      const int32_t *ridx, int k, float *da, hipStream_t stream);
+
+/* relu backward (dY *= Y > 0); column concatenation and its gradient split;
+ * X[i, :] += scale v. */
+int mbem_relu_bwd(const float *Y, float *dY, int64_t n, hipStream_t stream);
+int mbem_concat(const float *A, int64_t C1, const float *B, int64_t C2, int64_t P, float *out,
+                hipStream_t stream);
+int mbem_split(const float *d, int64_t C1, int64_t C2, int64_t P, float *dA, int accumulate_a,
+               float *dB, hipStream_t stream);
+int mbem_add_row(float *X, int64_t P, int64_t C, const float *v, float scale, hipStream_t stream);
+
+/* Relative loss |s y - t|^2 / |t|^2 (loss[0]) and dy (part: 2 x ceil(n/256)). */
+int mbem_loss(const float *y, const float *tgt, int64_t n, float s, float *part, float *dy,
+              float *loss, hipStream_t stream);
+
+/* Adam step with bias-corrected rate lr_t and decoupled weight decay wd. */
+int mbem_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr_t, float b1,
+              float b2, float eps, float wd, hipStream_t stream);
+
+/* Inference GEMM on the BF16 matrix cores: C (P x N) = act(A (P x K) W^T +
+ * bias), FP32 in and out, BF16 operands, FP32 accumulation; K % 16 == 0. */
+int mbem_hgemm(const float *A, int64_t lda, const float *W, int64_t K, int64_t N, int64_t P,
+               const float *bias, int relu, float *C, int64_t ldc, hipStream_t stream);
+
+/* Solver glue: point features from geometry (P x 9) and the FP64 residual r
+ * and block-Jacobi output z (3 unknowns per point), with their sums of
+ * squares in sumsq[0..1]; z += RMS(z) y; training target (e - z) / RMS(z). */
+int mbem_lp_features(const float *geo, int64_t P, const double *r, const double *z, double *sumsq,
+                     float *X, hipStream_t stream);
+int mbem_lp_apply_out(const float *y, int64_t P, const double *sumsq, double *z, hipStream_t stream);
+int mbem_lp_target(const double *e, const double *zbj, int64_t P, const double *sumsq, float *t,
+                   hipStream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

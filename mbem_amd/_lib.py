@@ -22,6 +22,7 @@ BJ_ROWS_PER_WAVE = 4
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
 _d = ctypes.c_double
+_f = ctypes.c_float
 _int = ctypes.c_int
 
 # name -> argument types (every entry returns int status unless listed below)
@@ -81,6 +82,23 @@ _SIGS = {
     "mbem_graph_launch": [_p, _p],
     "mbem_graph_destroy": [_p],
     "mbem_memcpy_async": [_p, _p, _i64, _p],
+    # learned preconditioner
+    "mbem_knn": [_p, _i64, _int, _p, _p],
+    "mbem_sgemm": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _i64, _i64, _f, _f, _p, _int,
+                   _i64, _p, _p],
+    "mbem_colsum": [_p, _i64, _i64, _p, _p, _i64, _f, _p],
+    "mbem_knn_mean": [_p, _i64, _i64, _p, _int, _p, _p],
+    "mbem_knn_mean_bwd": [_p, _i64, _i64, _p, _p, _int, _p, _p],
+    "mbem_relu_bwd": [_p, _p, _i64, _p],
+    "mbem_concat": [_p, _i64, _p, _i64, _i64, _p, _p],
+    "mbem_split": [_p, _i64, _i64, _i64, _p, _int, _p, _p],
+    "mbem_add_row": [_p, _i64, _i64, _p, _f, _p],
+    "mbem_loss": [_p, _p, _i64, _f, _p, _p, _p, _p],
+    "mbem_adam": [_p, _p, _p, _p, _i64, _f, _f, _f, _f, _f, _p],
+    "mbem_hgemm": [_p, _i64, _p, _i64, _i64, _i64, _p, _int, _p, _i64, _p],
+    "mbem_lp_features": [_p, _i64, _p, _p, _p, _p, _p],
+    "mbem_lp_apply_out": [_p, _i64, _p, _p, _p],
+    "mbem_lp_target": [_p, _p, _i64, _p, _p, _p],
 }
 _RESTYPE = {"mbem_multidot_work": _i64}
 

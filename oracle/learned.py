@@ -1,0 +1,119 @@
+"""CPU oracle of the learned preconditioner's network (test infrastructure
+only: imported by tests, never by the product path).
+
+The network has no counterpart in the reference; this file restates the
+network the GPU kernels implement (mbem_amd/csrc/learned.hip) in numpy,
+forward and backward written out by hand, so the GPU gradients can be
+checked against it, and this backward is itself pinned by central finite
+differences in tests/test_learned_oracle.py.
+
+Parameters (one flat float vector, this order):
+  W1 (128, 16)  b1 (128)   point layer on the 16 input features
+  W2 (128, 128) b2 (128)   message layer, averaged over the k neighbours
+  W3 (128, 256) b3 (128)   on [h1 | m]
+  W4 (128, 256) b4 (128)   on [h3 | mean(h3)]
+  W5 (3, 128)   b5 (3)     output
+"""
+from __future__ import annotations
+
+import numpy as np
+
+WIDTH = 128
+N_IN = 16
+SHAPES = [("W1", (WIDTH, N_IN)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)),
+          ("W3", (WIDTH, 2 * WIDTH)), ("b3", (WIDTH,)), ("W4", (WIDTH, 2 * WIDTH)),
+          ("b4", (WIDTH,)), ("W5", (3, WIDTH)), ("b5", (3,))]
+
+
+def n_params() -> int:
+    return int(sum(np.prod(s) for _, s in SHAPES))
+
+
+def unpack(theta):
+    out, o = {}, 0
+    for name, shp in SHAPES:
+        k = int(np.prod(shp))
+        out[name] = theta[o:o + k].reshape(shp)
+        o += k
+    return out
+
+
+def relu(x):
+    return np.maximum(x, 0.0)
+
+
+def forward(theta, X, nbr):
+    """(y (P, 3), cache) for features X (P, 16) and neighbours nbr (P, k)."""
+    p = unpack(theta)
+    P = X.shape[0]
+    h1 = relu(X @ p["W1"].T + p["b1"])
+    a = relu(h1 @ p["W2"].T + p["b2"])
+    m = a[nbr].mean(axis=1)
+    c1 = np.concatenate([h1, m], axis=1)
+    h3 = relu(c1 @ p["W3"].T + p["b3"])
+    g = h3.mean(axis=0)
+    W4a, W4b = p["W4"][:, :WIDTH], p["W4"][:, WIDTH:]
+    h4 = relu(h3 @ W4a.T + (W4b @ g + p["b4"]))
+    y = h4 @ p["W5"].T + p["b5"]
+    cache = dict(X=X, nbr=nbr, h1=h1, a=a, m=m, c1=c1, h3=h3, g=g, h4=h4, P=P)
+    return y, cache
+
+
+def loss(y, t, s):
+    """|s y - t|^2 / |t|^2"""
+    d = s * y - t
+    return float((d * d).sum() / max((t * t).sum(), 1e-300))
+
+
+def backward(theta, cache, y, t, s):
+    """Gradient of ``loss`` with respect to theta (flat, same order)."""
+    p = unpack(theta)
+    X, nbr, h1, a, c1, h3, g, h4, P = (cache[k] for k in
+                                       ("X", "nbr", "h1", "a", "c1", "h3", "g", "h4", "P"))
+    k = nbr.shape[1]
+    tt = max(float((t * t).sum()), 1e-300)
+    dy = 2.0 * s * (s * y - t) / tt
+    grads = {}
+    grads["W5"] = dy.T @ h4
+    grads["b5"] = dy.sum(axis=0)
+    dh4 = (dy @ p["W5"]) * (h4 > 0)
+    W4a, W4b = p["W4"][:, :WIDTH], p["W4"][:, WIDTH:]
+    dc = dh4.sum(axis=0)
+    grads["W4"] = np.concatenate([dh4.T @ h3, np.outer(dc, g)], axis=1)
+    grads["b4"] = dc
+    dg = W4b.T @ dc
+    dh3 = (dh4 @ W4a + dg[None, :] / P) * (h3 > 0)
+    grads["W3"] = dh3.T @ c1
+    grads["b3"] = dh3.sum(axis=0)
+    dc1 = dh3 @ p["W3"]
+    dh1 = dc1[:, :WIDTH].copy()
+    dm = dc1[:, WIDTH:]
+    da = np.zeros_like(a)
+    np.add.at(da, nbr.reshape(-1), np.repeat(dm, k, axis=0) / k)
+    da *= (a > 0)
+    grads["W2"] = da.T @ h1
+    grads["b2"] = da.sum(axis=0)
+    dh1 += da @ p["W2"]
+    dh1 *= (h1 > 0)
+    grads["W1"] = dh1.T @ X
+    grads["b1"] = dh1.sum(axis=0)
+    return np.concatenate([grads[name].reshape(-1) for name, _ in SHAPES])
+
+
+def knn(pts, k):
+    """(P, k) nearest neighbours, self excluded, ties by index (brute force)."""
+    P = pts.shape[0]
+    d = ((pts[:, None, :] - pts[None, :, :]) ** 2).sum(-1)
+    d[np.arange(P), np.arange(P)] = np.inf
+    order = np.lexsort((np.broadcast_to(np.arange(P), (P, P)), d), axis=1)
+    return order[:, :k]
+
+
+def adam_step(w, g, m, v, t, lr, b1=0.9, b2=0.999, eps=1e-8, wd=0.0):
+    """One Adam step (Kingma & Ba 2015) with bias correction folded into the rate."""
+    lr_t = lr * np.sqrt(1 - b2 ** t) / (1 - b1 ** t)
+    gt = g + wd * w
+    m = b1 * m + (1 - b1) * gt
+    v = b2 * v + (1 - b2) * gt * gt
+    w = w - lr_t * m / (np.sqrt(v) + eps)
+    return w, m, v
