@@ -333,14 +333,20 @@ int mbem_adam(float *w, const float *g, float *m, float *v, int64_t n, float lr_
 int mbem_hgemm(const float *A, int64_t lda, const float *W, int64_t K, int64_t N, int64_t P,
                const float *bias, int relu, float *C, int64_t ldc, hipStream_t stream);
 
-/* Solver glue: point features from geometry (P x 9) and the FP64 residual r
- * and block-Jacobi output z (3 unknowns per point), with their sums of
- * squares in sumsq[0..1]; z += RMS(z) y; training target (e - z) / RMS(z). */
-int mbem_lp_features(const float *geo, int64_t P, const double *r, const double *z, double *sumsq,
-                     float *X, hipStream_t stream);
-int mbem_lp_apply_out(const float *y, int64_t P, const double *sumsq, double *z, hipStream_t stream);
-int mbem_lp_target(const double *e, const double *zbj, int64_t P, const double *sumsq, float *t,
+/* Low-rank correction from the network output Y (P x 6K): Phi = Y[:, 0:3K],
+ * Psi = Y[:, 3K:6K] as 3P x K bases (row 3p + a = Y[p, a K + off ..]).
+ * C (B x K) = Bas^T V_b (V FP64 or FP32, rows ldv apart; part: nslices x B x K
+ * doubles); Out_b (+)= alpha Bas C_b; dY[:, off..] = sum_b U_b C_b^T; batch
+ * loss (1/B) sum_b |pred_b - t_b|^2 / |t_b|^2 with its gradient. */
+int mbem_lr_coef(const float *Y, int64_t P, int64_t K, int64_t off, const void *V, int64_t ldv,
+                 int64_t B, int v_fp64, double *part, int64_t nslices, double *C, hipStream_t stream);
+int mbem_lr_expand(const float *Y, int64_t P, int64_t K, int64_t off, const double *C, int64_t B,
+                   void *Out, int64_t ldo, int out_fp64, double alpha, int accumulate,
                    hipStream_t stream);
+int mbem_lr_outer(const float *U, int64_t ldu, const double *C, int64_t B, int64_t P, int64_t K,
+                  int64_t off, float *dY, hipStream_t stream);
+int mbem_lr_loss(const float *pred, const float *tgt, int64_t n, int64_t B, float *dpred,
+                 float *loss_b, hipStream_t stream);
 
 #ifdef __cplusplus
 }

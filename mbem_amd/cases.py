@@ -13,8 +13,10 @@ slip: the Burgers vector is the up-dip unit vector (the hanging wall, on the
 per patch.  Units: km, GPa (mu = 30 GPa = 3e10 Pa, nu = 0.25 -> lam = 30).
 
 n_cells = 16 gives N = 2,048 box triangles (n = 6,144 unknowns), n_cells =
-50 gives N = 20,000 (n = 60,000; the FP64 matrix is 28.8 GB), n_cells = 35
-gives N = 9,800 (n = 29,400).
+50 gives N = 20,000 (n = 60,000; the FP64 matrix is 28.8 GB), n_cells = 36
+gives N = 10,368 (n = 31,104).  ``random_box_fault`` draws the family the
+learned preconditioner is trained on: fault strike, dip, size and the box
+resolution varied.
 """
 from __future__ import annotations
 
@@ -39,23 +41,31 @@ def box_meshes(n_cells: int, half=20.0, depth=20.0):
     return top, sides, base
 
 
-def dipping_fault(size=10.0, cells=(10, 20), dip=45.0, top_z=-3.0):
-    """(mesh, normal, burgers): the fault plane dipping toward +x."""
+def dipping_fault(size=10.0, cells=(10, 20), dip=45.0, top_z=-3.0, strike=0.0):
+    """(mesh, normal, burgers): the fault plane dipping toward +x, then turned
+    by ``strike`` degrees about the vertical through the origin."""
     vert, _, _ = vertical_fault(size, (-size, 0.0), cells[0], cells[1])
     mesh, normal = tilted_fault(vert, dip)
     v = mesh.vertices.copy()
     xs = v[:, 0]
     v[:, 0] -= 0.5 * (xs.min() + xs.max())
     v[:, 2] += top_z - v[:, 2].max()
-    mesh.vertices = v
     d = np.deg2rad(dip)
     burgers = np.array([-np.cos(d), 0.0, np.sin(d)])
+    if strike:
+        a = np.deg2rad(strike)
+        R = np.array([[np.cos(a), -np.sin(a), 0.0], [np.sin(a), np.cos(a), 0.0], [0.0, 0.0, 1.0]])
+        v = v @ R.T
+        normal = R @ np.asarray(normal, float)
+        burgers = R @ burgers
+    mesh.vertices = v
     return mesh, normal, burgers
 
 
 def box_fault_model(n_cells: int = 16, fault_cells=(10, 20), dip: float = 45.0,
                     mu: float = 30.0, lam: float = 30.0, eps_over_h: float = 0.5,
-                    slip: float = 1.0, fault: bool = True):
+                    slip: float = 1.0, fault: bool = True, strike: float = 0.0,
+                    fault_size: float = 10.0):
     """(model, eps spec, info) of the box-with-fault workload."""
     top, sides, base = box_meshes(n_cells)
     p_top = Patch("top", top, BCType.FREE_TRACTION)
@@ -68,7 +78,8 @@ def box_fault_model(n_cells: int = 16, fault_cells=(10, 20), dip: float = 45.0,
     info = {"n_cells": n_cells, "box_triangles": top.n_triangles + sides.n_triangles
             + base.n_triangles}
     if fault:
-        fmesh, fnormal, burgers = dipping_fault(cells=fault_cells, dip=dip)
+        fmesh, fnormal, burgers = dipping_fault(size=fault_size, cells=fault_cells, dip=dip,
+                                                strike=strike)
         p_fault = Patch("fault", fmesh, BCType.FAULT,
                         value=np.broadcast_to(slip * burgers, (fmesh.n_triangles, 3)))
         faults.append(p_fault)
@@ -78,6 +89,24 @@ def box_fault_model(n_cells: int = 16, fault_cells=(10, 20), dip: float = 45.0,
     region = Region("crust", ElasticMaterial(mu=mu, lam=lam), [p_top, p_sides, p_base],
                     probe_point=np.array([9.0, 9.0, -10.0]), faults=faults)
     return RegionModel([region]), eps, info
+
+
+def random_box_fault(rng, n_cells_range=(26, 50)):
+    """(model, eps, info) of one random member of the training family: box
+    resolution n_cells even in ``n_cells_range`` (N = 8 n^2 triangles: 5.4k
+    to 20k for the default), fault strike U(0, 180) deg, dip U(20, 90) deg,
+    size U(5, 15) km with cells matched to the box spacing."""
+    lo, hi = n_cells_range
+    n_cells = int(rng.integers(lo // 2, hi // 2 + 1)) * 2
+    strike = float(rng.uniform(0.0, 180.0))
+    dip = float(rng.uniform(20.0, 90.0))
+    size = float(rng.uniform(5.0, 15.0))
+    h = 40.0 / n_cells
+    nf = max(2, int(round(size / h)))
+    model, eps, info = box_fault_model(n_cells, fault_cells=(nf, 2 * nf), dip=dip, strike=strike,
+                                       fault_size=size)
+    info.update(strike=strike, dip=dip, size=size)
+    return model, eps, info
 
 
 def surface_grid(n=50, half=20.0, z=0.0):

@@ -7,7 +7,9 @@
 //   h3 = relu([h1 | m] W3^T + b3)               P x 128
 //   g  = mean_i h3_i                            128
 //   h4 = relu(h3 W4a^T + (W4b g + b4))          P x 128
-//   y  = h4 W5^T + b5                           P x 3
+//   y  = h4 W5^T + b5                           P x 6K
+// The output holds two 3P x K bases Phi, Psi of a low-rank correction
+// Phi (Psi^T r) added to the block-Jacobi preconditioner (k_lr_* below).
 // Training: FP32 throughout (VALU SGEMM below, split-K for weight gradients,
 // deterministic reductions, Adam).  Inference: the same graph with the dense
 // layers on the BF16 matrix cores (k_hgemm), FP32 accumulation.
@@ -396,81 +398,106 @@ __global__ __launch_bounds__(256) void k_hgemm(const float* __restrict__ A, long
 }
 
 // ---------------------------------------------------------------------------
-// glue between the solver's FP64 vectors (3 unknowns per point, point p owns
-// unknowns 3p..3p+2) and the network's FP32 point features
+// Low-rank correction of the learned preconditioner.  The network's output
+// Y (P x 6K, row-major) holds, for point p and component a, the rows
+//   Phi[3p + a, :] = Y[p, a K .. a K + K - 1]
+//   Psi[3p + a, :] = Y[p, 3K + a K .. 3K + a K + K - 1]
+// of two 3P x K bases; the correction is Phi (Psi^T r).
 
-// X[p, 0:9] = geometry features (given), X[p, 9:12] = r_p / sr, X[p, 12:15] =
-// z_p / sz, X[p, 15] = 0, with sr, sz the RMS of r and z per point read from
-// norms[0], norms[1] (sums of squares) on the device.
-__global__ __launch_bounds__(256) void k_features(const float* __restrict__ geo, long P,
-                                                  const double* __restrict__ r,
-                                                  const double* __restrict__ z,
-                                                  const double* __restrict__ sumsq,
-                                                  float* __restrict__ X)
+// C[b, k] = sum_i Bas[i, k] V[b, i] over i < 3P (Bas = Phi or Psi through its
+// column offset off = 0 or 3K); partial sums over point slices (FP64 accumulation).
+template <typename T>
+__global__ __launch_bounds__(256) void k_lr_coef_part(const float* __restrict__ Y, long P, long K,
+                                                      long off, const T* __restrict__ V, long ldv,
+                                                      long B, long pts_per, double* __restrict__ part)
 {
-    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= P) return;
-    const double sr = sqrt(sumsq[0] / (double)P), sz = sqrt(sumsq[1] / (double)P);
-    const double ir = sr > 0.0 ? 1.0 / sr : 0.0, iz = sz > 0.0 ? 1.0 / sz : 0.0;
-    float* x = X + p * 16;
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;   // (b, k) pair
+    if (q >= B * K) return;
+    const long b = q / K, k = q % K;
+    const long p0 = (long)blockIdx.y * pts_per;
+    const long p1 = (p0 + pts_per < P) ? p0 + pts_per : P;
+    const long W = 6 * K;
+    double acc = 0.0;
+    for (long p = p0; p < p1; ++p) {
 #pragma unroll
-    for (int q = 0; q < 9; ++q) x[q] = geo[p * 9 + q];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        x[9 + a] = (float)(r[3 * p + a] * ir);
-        x[12 + a] = (float)(z[3 * p + a] * iz);
+        for (int a = 0; a < 3; ++a)
+            acc += (double)Y[p * W + off + a * K + k] * (double)V[b * ldv + 3 * p + a];
     }
-    x[15] = 0.f;
+    part[(long)blockIdx.y * B * K + q] = acc;
 }
 
-// sums of squares of two FP64 vectors (one block; n small enough)
-__global__ __launch_bounds__(256) void k_sumsq2(const double* __restrict__ a,
-                                                const double* __restrict__ b, long n,
-                                                double* __restrict__ out)
+__global__ __launch_bounds__(256) void k_lr_coef_sum(const double* __restrict__ part, long ns,
+                                                     long BK, double* __restrict__ C)
 {
-    __shared__ double s0[256], s1[256];
-    double x = 0.0, y = 0.0;
-    for (long t = threadIdx.x; t < n; t += 256) {
-        x += a[t] * a[t];
-        y += b[t] * b[t];
+    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= BK) return;
+    double s = 0.0;
+    for (long z = 0; z < ns; ++z) s += part[z * BK + q];
+    C[q] = s;
+}
+
+// Out[b, 3p + a] (+)= alpha sum_k Y[p, off + a K + k] C[b, k]
+template <typename T>
+__global__ __launch_bounds__(256) void k_lr_expand(const float* __restrict__ Y, long P, long K,
+                                                   long off, const double* __restrict__ C, long B,
+                                                   T* __restrict__ Out, long ldo, double alpha,
+                                                   int accumulate)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= B * 3 * P) return;
+    const long b = t / (3 * P), i = t % (3 * P);
+    const long p = i / 3, a = i % 3;
+    const float* y = Y + p * 6 * K + off + a * K;
+    const double* c = C + b * K;
+    double s = 0.0;
+    for (long k = 0; k < K; ++k) s += (double)y[k] * c[k];
+    T* o = Out + b * ldo + i;
+    *o = accumulate ? (T)((double)*o + alpha * s) : (T)(alpha * s);
+}
+
+// dY[p, off + a K + k] = sum_b U[b, 3p + a] C[b, k]  (the outer-product gradient)
+__global__ __launch_bounds__(256) void k_lr_outer(const float* __restrict__ U, long ldu,
+                                                  const double* __restrict__ C, long B, long P,
+                                                  long K, long off, float* __restrict__ dY)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * P * K) return;
+    const long i = t / K, k = t % K;
+    const long p = i / 3, a = i % 3;
+    double s = 0.0;
+    for (long b = 0; b < B; ++b) s += (double)U[b * ldu + i] * C[b * K + k];
+    dY[p * 6 * K + off + a * K + k] = (float)s;
+}
+
+// batch loss: L = (1/B) sum_b |pred_b - t_b|^2 / |t_b|^2 and dpred (one block per b)
+__global__ __launch_bounds__(256) void k_lr_loss(const float* __restrict__ pred,
+                                                 const float* __restrict__ tgt, long n, long B,
+                                                 float* __restrict__ dpred, float* __restrict__ lb)
+{
+    __shared__ double r0[256], r1[256];
+    const long b = blockIdx.x;
+    const float* pr = pred + b * n;
+    const float* tg = tgt + b * n;
+    double d2 = 0.0, t2 = 0.0;
+    for (long i = threadIdx.x; i < n; i += 256) {
+        const double d = (double)pr[i] - (double)tg[i];
+        d2 += d * d;
+        t2 += (double)tg[i] * tg[i];
     }
-    s0[threadIdx.x] = x;
-    s1[threadIdx.x] = y;
+    r0[threadIdx.x] = d2;
+    r1[threadIdx.x] = t2;
     __syncthreads();
     for (int w = 128; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
-            s0[threadIdx.x] += s0[threadIdx.x + w];
-            s1[threadIdx.x] += s1[threadIdx.x + w];
+            r0[threadIdx.x] += r0[threadIdx.x + w];
+            r1[threadIdx.x] += r1[threadIdx.x + w];
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) {
-        out[0] = s0[0];
-        out[1] = s1[0];
-    }
-}
-
-// z[3p + a] += (sz) * y[p, a]   (sz = RMS of the block-Jacobi output per point)
-__global__ __launch_bounds__(256) void k_apply_out(const float* __restrict__ y, long P,
-                                                   const double* __restrict__ sumsq,
-                                                   double* __restrict__ z)
-{
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= 3 * P) return;
-    const double sz = sqrt(sumsq[1] / (double)P);
-    z[t] += sz * (double)y[t];
-}
-
-// target of a training pair: t[p, a] = (e - zbj)[3p + a] / sz  (FP32)
-__global__ __launch_bounds__(256) void k_target(const double* __restrict__ e,
-                                                const double* __restrict__ zbj, long P,
-                                                const double* __restrict__ sumsq,
-                                                float* __restrict__ t)
-{
-    const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (q >= 3 * P) return;
-    const double sz = sqrt(sumsq[1] / (double)P);
-    t[q] = (float)((e[q] - zbj[q]) / (sz > 0.0 ? sz : 1.0));
+    const double tt = r1[0] > 0.0 ? r1[0] : 1.0;
+    for (long i = threadIdx.x; i < n; i += 256)
+        dpred[b * n + i] = (float)(2.0 * ((double)pr[i] - (double)tg[i]) / (tt * (double)B));
+    if (threadIdx.x == 0) lb[b] = (float)(r0[0] / tt / (double)B);
 }
 
 }  // namespace
@@ -605,30 +632,56 @@ int mbem_hgemm(const float* A, int64_t lda, const float* W, int64_t K, int64_t N
     return launch_status();
 }
 
-int mbem_lp_features(const float* geo, int64_t P, const double* r, const double* z, double* sumsq,
-                     float* X, hipStream_t stream)
+int mbem_lr_coef(const float* Y, int64_t P, int64_t K, int64_t off, const void* V, int64_t ldv,
+                 int64_t B, int v_fp64, double* part, int64_t nslices, double* C, hipStream_t stream)
 {
-    if (P < 1 || !geo || !r || !z || !sumsq || !X) return MBEM_EINVAL;
-    hipLaunchKernelGGL(k_sumsq2, dim3(1), dim3(256), 0, stream, r, z, 3 * P, sumsq);
-    hipLaunchKernelGGL(k_features, dim3((unsigned)((P + 255) / 256)), dim3(256), 0, stream, geo, P, r, z,
-                       sumsq, X);
+    if (P < 1 || K < 1 || B < 1 || nslices < 1 || nslices > 65535 || !Y || !V || !part || !C)
+        return MBEM_EINVAL;
+    if (off != 0 && off != 3 * K) return MBEM_EINVAL;
+    const long per = (P + nslices - 1) / nslices;
+    dim3 grid((unsigned)((B * K + 255) / 256), (unsigned)nslices);
+    if (v_fp64)
+        hipLaunchKernelGGL(k_lr_coef_part<double>, grid, dim3(256), 0, stream, Y, P, K, off,
+                           (const double*)V, ldv, B, per, part);
+    else
+        hipLaunchKernelGGL(k_lr_coef_part<float>, grid, dim3(256), 0, stream, Y, P, K, off,
+                           (const float*)V, ldv, B, per, part);
+    hipLaunchKernelGGL(k_lr_coef_sum, dim3((unsigned)((B * K + 255) / 256)), dim3(256), 0, stream, part,
+                       nslices, B * K, C);
     return launch_status();
 }
 
-int mbem_lp_apply_out(const float* y, int64_t P, const double* sumsq, double* z, hipStream_t stream)
-{
-    if (P < 1 || !y || !sumsq || !z) return MBEM_EINVAL;
-    hipLaunchKernelGGL(k_apply_out, dim3((unsigned)((3 * P + 255) / 256)), dim3(256), 0, stream, y, P,
-                       sumsq, z);
-    return launch_status();
-}
-
-int mbem_lp_target(const double* e, const double* zbj, int64_t P, const double* sumsq, float* t,
+int mbem_lr_expand(const float* Y, int64_t P, int64_t K, int64_t off, const double* C, int64_t B,
+                   void* Out, int64_t ldo, int out_fp64, double alpha, int accumulate,
                    hipStream_t stream)
 {
-    if (P < 1 || !e || !zbj || !sumsq || !t) return MBEM_EINVAL;
-    hipLaunchKernelGGL(k_target, dim3((unsigned)((3 * P + 255) / 256)), dim3(256), 0, stream, e, zbj, P,
-                       sumsq, t);
+    if (P < 1 || K < 1 || B < 1 || !Y || !C || !Out) return MBEM_EINVAL;
+    if (off != 0 && off != 3 * K) return MBEM_EINVAL;
+    const long n = B * 3 * P;
+    if (out_fp64)
+        hipLaunchKernelGGL(k_lr_expand<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, Y,
+                           P, K, off, C, B, (double*)Out, ldo, alpha, accumulate);
+    else
+        hipLaunchKernelGGL(k_lr_expand<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, Y,
+                           P, K, off, C, B, (float*)Out, ldo, alpha, accumulate);
+    return launch_status();
+}
+
+int mbem_lr_outer(const float* U, int64_t ldu, const double* C, int64_t B, int64_t P, int64_t K,
+                  int64_t off, float* dY, hipStream_t stream)
+{
+    if (P < 1 || K < 1 || B < 1 || !U || !C || !dY) return MBEM_EINVAL;
+    if (off != 0 && off != 3 * K) return MBEM_EINVAL;
+    hipLaunchKernelGGL(k_lr_outer, dim3((unsigned)((3 * P * K + 255) / 256)), dim3(256), 0, stream, U, ldu,
+                       C, B, P, K, off, dY);
+    return launch_status();
+}
+
+int mbem_lr_loss(const float* pred, const float* tgt, int64_t n, int64_t B, float* dpred,
+                 float* loss_b, hipStream_t stream)
+{
+    if (n < 1 || B < 1 || B > 65535 || !pred || !tgt || !dpred || !loss_b) return MBEM_EINVAL;
+    hipLaunchKernelGGL(k_lr_loss, dim3((unsigned)B), dim3(256), 0, stream, pred, tgt, n, B, dpred, loss_b);
     return launch_status();
 }
 

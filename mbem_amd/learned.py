@@ -1,44 +1,42 @@
 """Learned FGMRES preconditioner: a point-cloud network over the boundary
-elements that corrects the block-Jacobi preconditioner's output.
+elements that proposes a low-rank correction of block-Jacobi.
 
-    M(r) = BJ(r) + RMS(BJ(r)) * net(features(r, BJ(r)))
+    M(r) = BJ(r) + Phi (Psi^T r),   Phi, Psi: n x K  (K = 16)
 
 One point per boundary triangle (its three unknowns are one slot triple).
-Per-point inputs (16): the centroid (scaled to the unit box of the mesh), the
-normal, log(area / mean area), the slot kind (u or t, one-hot), the residual
-and the block-Jacobi output at the point, each divided by its RMS over the
-points, and a zero pad.  Network (width 128): a point layer, one k-nearest-
-neighbour message step (k = 16, mean of transformed neighbours), a layer on
-[h1 | m], a layer on [h3 | mean over all points of h3] (global context), and
-a linear 3-output head; ReLU between.  FGMRES tolerates a preconditioner
-that is not linear in r: the Arnoldi relation uses the stored z vectors
-(solver.py:115-245, flexible variant).
+The network sees geometry only — per point the centroid (scaled to the unit
+box of the mesh), the normal, log(area / mean area) and the slot kind (u or
+t, one-hot), zero-padded to 16 inputs — so M is linear in r and fixed for a
+mesh: the network runs once per mesh, and each FGMRES application costs the
+block-Jacobi apply plus two n x K products.  Network (width 128): a point
+layer, one k-nearest-neighbour message step (k = 16, mean of transformed
+neighbours), a layer on [h1 | m], a layer on [h3 | mean over all points of
+h3] (global context), and a linear head with 6K outputs per point, the rows
+of Phi and Psi at the point's three unknowns; ReLU between.
 
 Training: residual/correction pairs from our own solves.  For a system
 A x = b, restarted FGMRES cycles with block-Jacobi give iterates x_c; with
-the converged x*, r_c = b - A x_c and e_c = x* - x_c satisfy A e_c = r_c, and
-the network learns the part of e_c that block-Jacobi misses,
-(e_c - BJ(r_c)) / RMS(BJ(r_c)), under the loss |y - t|^2 / |t|^2.  All of
-it — features, forward, backward, Adam — runs in the HIP kernels of
-csrc/learned.hip (FP32); inference runs the dense layers on the BF16 matrix
-cores (``mbem_hgemm``).  The network has no counterpart in the reference;
-block-Jacobi FGMRES (preconditioner.py:314-543) is its fallback and the
-measure of its gain.
+the converged x*, r_c = b - A x_c and e_c = x* - x_c satisfy A e_c = r_c.  The
+network learns the part of e_c that block-Jacobi misses: with r^ = r / |r|
+and t^ = (e - BJ(r)) / |r|, the loss over a geometry's pairs is
+(1/B) sum_b |Phi Psi^T r^_b - t^_b|^2 / |t^_b|^2.  Features, forward,
+backward and Adam run in the HIP kernels of csrc/learned.hip (FP32);
+inference runs the dense layers on the BF16 matrix cores (``mbem_hgemm``).
+The network has no counterpart in the reference; block-Jacobi FGMRES
+(preconditioner.py:314-543) is its fallback and the measure of its gain.
 """
 from __future__ import annotations
 
-import ctypes
 import math
 import os
 import time
-from dataclasses import dataclass, field
+from dataclasses import dataclass
 
 import numpy as np
 import torch
 
 from . import _lib
 from ._lib import call, ptr, stream
-from .kernels import basis as kb
 
 F32 = torch.float32
 F64 = torch.float64
@@ -48,9 +46,11 @@ WIDTH = 128
 N_IN = 16
 N_GEO = 9
 K_NBR = 16
+RANK = 16
+N_OUT = 6 * RANK
 SHAPES = [("W1", (WIDTH, N_IN)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)),
           ("W3", (WIDTH, 2 * WIDTH)), ("b3", (WIDTH,)), ("W4", (WIDTH, 2 * WIDTH)),
-          ("b4", (WIDTH,)), ("W5", (3, WIDTH)), ("b5", (3,))]
+          ("b4", (WIDTH,)), ("W5", (N_OUT, WIDTH)), ("b5", (N_OUT,))]
 WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "learned_precond.npz")
 
 
@@ -59,15 +59,15 @@ def n_params() -> int:
 
 
 def init_params(seed: int = 0) -> np.ndarray:
-    """He-uniform weights, zero biases; the head starts at zero so an
-    untrained network leaves block-Jacobi unchanged."""
+    """He-uniform weights, zero biases, a small head (the correction starts
+    near zero, so an untrained network is close to block-Jacobi)."""
     rng = np.random.default_rng(seed)
     out = []
     for name, shp in SHAPES:
-        if name.startswith("b") or name == "W5":
+        if name.startswith("b"):
             out.append(np.zeros(shp))
         else:
-            lim = math.sqrt(6.0 / shp[1])
+            lim = math.sqrt(6.0 / shp[1]) * (0.05 if name == "W5" else 1.0)
             out.append(rng.uniform(-lim, lim, size=shp))
     return np.concatenate([a.reshape(-1) for a in out]).astype(np.float32)
 
@@ -104,6 +104,8 @@ class Cloud:
         self.P = geo.shape[0]
         self.dev = dev
         self.geo = torch.as_tensor(geo, device=dev).contiguous()
+        feats = np.zeros((self.P, N_IN), dtype=np.float32)
+        feats[:, :N_GEO] = geo
         pts = torch.as_tensor(cen.astype(np.float32), device=dev).contiguous()
         self.nbr = torch.empty((self.P, K_NBR), dtype=I32, device=dev)
         call("mbem_knn", ptr(pts), self.P, K_NBR, ptr(self.nbr), stream())
@@ -116,15 +118,14 @@ class Cloud:
         self.ridx = torch.as_tensor(src.astype(np.int32), device=dev)
         P = self.P
         e = dict(dtype=F32, device=dev)
-        self.X = torch.zeros((P, N_IN), **e)
+        self.X = torch.as_tensor(feats, device=dev).contiguous()
         self.h1 = torch.zeros((P, WIDTH), **e)
         self.a = torch.zeros((P, WIDTH), **e)
         self.m = torch.zeros((P, WIDTH), **e)
         self.c1 = torch.zeros((P, 2 * WIDTH), **e)
         self.h3 = torch.zeros((P, WIDTH), **e)
         self.h4 = torch.zeros((P, WIDTH), **e)
-        self.y = torch.zeros((P, 3), **e)
-        self.sumsq = torch.zeros(2, dtype=F64, device=dev)
+        self.y = torch.zeros((P, N_OUT), **e)
 
 
 class PointNet:
@@ -241,11 +242,12 @@ class PointNet:
             call("mbem_colsum", ptr(dY), P, dY.shape[1], ptr(Gb), ptr(tmp), ns, 1.0, stream())
 
         # head
-        wgrad(dy, cl.h4, WIDTH, self.G("W5"), 3)
+        nout = dy.shape[1]
+        wgrad(dy, cl.h4, WIDTH, self.G("W5"), nout)
         bgrad(dy, self.G("b5"))
         dh4 = torch.empty((P, WIDTH), **e)
-        call("mbem_sgemm", ptr(dy), 3, 1, ptr(self.W("W5")), WIDTH, 1, ptr(dh4), WIDTH, P, WIDTH, 3,
-             1.0, 0.0, None, 0, 1, None, stream())
+        call("mbem_sgemm", ptr(dy), nout, 1, ptr(self.W("W5")), WIDTH, 1, ptr(dh4), WIDTH, P, WIDTH,
+             nout, 1.0, 0.0, None, 0, 1, None, stream())
         call("mbem_relu_bwd", ptr(cl.h4), ptr(dh4), P * WIDTH, stream())
         # layer 4: local half, bias, global half (dW4b = dc g^T / ..., dg = W4b^T dc)
         G4 = self.G("W4")
@@ -295,12 +297,39 @@ class PointNet:
         return self._work2
 
     def loss_grad(self, cl: Cloud, tgt, dy, s: float = 1.0):
-        """dy = d(|s y - t|^2/|t|^2)/dy; the loss value stays on the device."""
-        n = 3 * cl.P
+        """dy = d(|s y - t|^2/|t|^2)/dy over the whole output; the loss value
+        stays on the device."""
+        n = cl.y.numel()
         part = self._ws2(2 * ((n + 255) // 256))
         call("mbem_loss", ptr(cl.y), ptr(tgt), n, float(s), ptr(part), ptr(dy), ptr(self._loss),
              stream())
         return self._loss
+
+    def lowrank_loss_grad(self, cl: Cloud, R, T, dy):
+        """Batch loss (1/B) sum_b |Phi Psi^T r_b - t_b|^2 / |t_b|^2 of the
+        output cl.y and its gradient dy (P x 6K); R, T: B x 3P FP32."""
+        B, n = R.shape
+        P, K = cl.P, RANK
+        ns = min(256, max(1, P // 64))
+        part = self._part(ns * B * K)
+        C = torch.empty((B, K), dtype=F64, device=self.dev)
+        dC = torch.empty((B, K), dtype=F64, device=self.dev)
+        pred = torch.empty((B, n), dtype=F32, device=self.dev)
+        dpred = torch.empty((B, n), dtype=F32, device=self.dev)
+        lb = torch.empty(B, dtype=F32, device=self.dev)
+        call("mbem_lr_coef", ptr(cl.y), P, K, 3 * K, ptr(R), n, B, 0, ptr(part), ns, ptr(C), stream())
+        call("mbem_lr_expand", ptr(cl.y), P, K, 0, ptr(C), B, ptr(pred), n, 0, 1.0, 0, stream())
+        call("mbem_lr_loss", ptr(pred), ptr(T), n, B, ptr(dpred), ptr(lb), stream())
+        call("mbem_lr_outer", ptr(dpred), n, ptr(C), B, P, K, 0, ptr(dy), stream())
+        call("mbem_lr_coef", ptr(cl.y), P, K, 0, ptr(dpred), n, B, 0, ptr(part), ns, ptr(dC),
+             stream())
+        call("mbem_lr_outer", ptr(R), n, ptr(dC), B, P, K, 3 * K, ptr(dy), stream())
+        return lb.sum()
+
+    def _part(self, n):
+        if not hasattr(self, "_part_t") or self._part_t.numel() < n:
+            self._part_t = torch.zeros(max(n, 1), dtype=F64, device=self.dev)
+        return self._part_t
 
     def zero_grad(self):
         self.grad.zero_()
@@ -316,23 +345,34 @@ class PointNet:
 
 # ---------------------------------------------------------------------------
 class LearnedPreconditioner:
-    """z = BJ(r) + RMS(BJ(r)) net(r, BJ(r)): a right preconditioner for FGMRES
-    (``apply_into(r, z)``), inference on the BF16 matrix cores."""
+    """z = BJ(r) + Phi (Psi^T r): a linear right preconditioner for FGMRES
+    (``apply_into(r, z)``).  The network runs once, on the BF16 matrix cores
+    (or FP32), when the preconditioner is made; each application adds two
+    n x K products to the block-Jacobi apply."""
 
     def __init__(self, base, layout, net: PointNet = None, bf16: bool = True):
         self.base = base
         self.cloud = Cloud(layout)
         self.net = net or PointNet.load()
         self.bf16 = bf16
+        t0 = time.perf_counter()
+        self.net.forward(self.cloud, bf16=bf16)
+        self.Y = self.cloud.y.clone()
+        torch.cuda.current_stream().synchronize()
+        self.infer_s = time.perf_counter() - t0
+        P = self.cloud.P
+        self.ns = min(256, max(1, P // 64))
+        self._part = torch.zeros(self.ns * RANK, dtype=F64, device=self.cloud.dev)
+        self._c = torch.zeros(RANK, dtype=F64, device=self.cloud.dev)
         self.applies = 0
 
     def apply_into(self, r, z):
         self.base.apply_into(r, z)
-        cl = self.cloud
-        call("mbem_lp_features", ptr(cl.geo), cl.P, ptr(r), ptr(z), ptr(cl.sumsq), ptr(cl.X),
+        P, K = self.cloud.P, RANK
+        call("mbem_lr_coef", ptr(self.Y), P, K, 3 * K, ptr(r), 3 * P, 1, 1, ptr(self._part), self.ns,
+             ptr(self._c), stream())
+        call("mbem_lr_expand", ptr(self.Y), P, K, 0, ptr(self._c), 1, ptr(z), 3 * P, 1, 1.0, 1,
              stream())
-        self.net.forward(cl, bf16=self.bf16)
-        call("mbem_lp_apply_out", ptr(cl.y), cl.P, ptr(cl.sumsq), ptr(z), stream())
         self.applies += 1
         return z
 
@@ -342,7 +382,8 @@ class LearnedPreconditioner:
 
     def summary(self):
         s = dict(self.base.summary()) if hasattr(self.base, "summary") else {}
-        s.update(learned=True, points=self.cloud.P, bf16=self.bf16)
+        s.update(learned=True, points=self.cloud.P, rank=RANK, bf16=self.bf16,
+                 inference_s=self.infer_s)
         return s
 
 
@@ -352,8 +393,8 @@ class LearnedPreconditioner:
 @dataclass
 class Sample:
     cloud: Cloud
-    X: torch.Tensor          # P x 16 features
-    t: torch.Tensor          # P x 3 target
+    R: torch.Tensor          # B x 3P  normalised residuals r / |r|
+    T: torch.Tensor          # B x 3P  targets (e - BJ(r)) / |r|
 
 
 def residual_pairs(asm, M, cycles: int = 8, restart: int = 10, rtol: float = 1e-12):
@@ -375,18 +416,17 @@ def residual_pairs(asm, M, cycles: int = 8, restart: int = 10, rtol: float = 1e-
 
 
 def make_samples(asm, M, cloud: Cloud, pairs):
-    """Features and targets of each (r, e) pair (uses the block-Jacobi M)."""
-    out = []
+    """One batch Sample of a geometry's (r, e) pairs (uses block-Jacobi M)."""
+    R, T = [], []
     for r, e in pairs:
+        nr = float(torch.linalg.vector_norm(r))
+        if nr == 0.0:
+            continue
         z = torch.empty_like(r)
         M.apply_into(r, z)
-        X = torch.empty((cloud.P, N_IN), dtype=F32, device=cloud.dev)
-        call("mbem_lp_features", ptr(cloud.geo), cloud.P, ptr(r), ptr(z), ptr(cloud.sumsq), ptr(X),
-             stream())
-        t = torch.empty((cloud.P, 3), dtype=F32, device=cloud.dev)
-        call("mbem_lp_target", ptr(e), ptr(z), cloud.P, ptr(cloud.sumsq), ptr(t), stream())
-        out.append(Sample(cloud, X, t))
-    return out
+        R.append((r / nr).float())
+        T.append(((e - z) / nr).float())
+    return [Sample(cloud, torch.stack(R).contiguous(), torch.stack(T).contiguous())]
 
 
 def train(net: PointNet, samples, epochs: int = 10, lr: float = 1e-3, seed: int = 0,
@@ -402,10 +442,9 @@ def train(net: PointNet, samples, epochs: int = 10, lr: float = 1e-3, seed: int 
         for i in order:
             s = samples[i]
             cl = s.cloud
-            cl.X.copy_(s.X)
             net.forward(cl)
             dy = torch.empty_like(cl.y)
-            loss = net.loss_grad(cl, s.t, dy)
+            loss = net.lowrank_loss_grad(cl, s.R, s.T, dy)
             tot += loss
             net.zero_grad()
             net.backward(cl, dy)
@@ -420,8 +459,7 @@ def evaluate_loss(net: PointNet, samples) -> float:
     tot = 0.0
     for s in samples:
         cl = s.cloud
-        cl.X.copy_(s.X)
         net.forward(cl)
         dy = torch.empty_like(cl.y)
-        tot += float(net.loss_grad(cl, s.t, dy).item())
+        tot += float(net.lowrank_loss_grad(cl, s.R, s.T, dy).item())
     return tot / max(len(samples), 1)

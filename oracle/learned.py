@@ -12,7 +12,12 @@ Parameters (one flat float vector, this order):
   W2 (128, 128) b2 (128)   message layer, averaged over the k neighbours
   W3 (128, 256) b3 (128)   on [h1 | m]
   W4 (128, 256) b4 (128)   on [h3 | mean(h3)]
-  W5 (3, 128)   b5 (3)     output
+  W5 (6K, 128)  b5 (6K)    output: two 3P x K bases Phi, Psi (K = 16)
+
+The preconditioner built from the output is z = BJ(r) + Phi (Psi^T r), with
+Phi[3p + a, k] = y[p, a K + k] and Psi[3p + a, k] = y[p, 3K + a K + k]; the
+training loss over a batch of normalised pairs (r_b, t_b) is
+(1/B) sum_b |Phi Psi^T r_b - t_b|^2 / |t_b|^2 (``lowrank_loss``).
 """
 from __future__ import annotations
 
@@ -20,9 +25,11 @@ import numpy as np
 
 WIDTH = 128
 N_IN = 16
+RANK = 16
+N_OUT = 6 * RANK
 SHAPES = [("W1", (WIDTH, N_IN)), ("b1", (WIDTH,)), ("W2", (WIDTH, WIDTH)), ("b2", (WIDTH,)),
           ("W3", (WIDTH, 2 * WIDTH)), ("b3", (WIDTH,)), ("W4", (WIDTH, 2 * WIDTH)),
-          ("b4", (WIDTH,)), ("W5", (3, WIDTH)), ("b5", (3,))]
+          ("b4", (WIDTH,)), ("W5", (N_OUT, WIDTH)), ("b5", (N_OUT,))]
 
 
 def n_params() -> int:
@@ -65,14 +72,42 @@ def loss(y, t, s):
     return float((d * d).sum() / max((t * t).sum(), 1e-300))
 
 
-def backward(theta, cache, y, t, s):
-    """Gradient of ``loss`` with respect to theta (flat, same order)."""
+def bases(y):
+    """(Phi, Psi), each 3P x K, from the network output y (P x 6K)."""
+    P = y.shape[0]
+    K = y.shape[1] // 6
+    return y[:, :3 * K].reshape(3 * P, K), y[:, 3 * K:].reshape(3 * P, K)
+
+
+def lowrank_loss(y, R, T):
+    """(loss, dy) of (1/B) sum_b |Phi Psi^T r_b - t_b|^2 / |t_b|^2; R, T: B x 3P."""
+    Phi, Psi = bases(y)
+    B = R.shape[0]
+    C = R @ Psi                                  # B x K
+    pred = C @ Phi.T                             # B x 3P
+    tt = np.maximum((T * T).sum(axis=1), 1e-300)
+    d = pred - T
+    loss = float(((d * d).sum(axis=1) / tt).sum() / B)
+    dpred = 2.0 * d / (tt[:, None] * B)
+    dPhi = dpred.T @ C                           # 3P x K
+    dC = dpred @ Phi                             # B x K
+    dPsi = R.T @ dC                              # 3P x K
+    P = y.shape[0]
+    K = y.shape[1] // 6
+    dy = np.concatenate([dPhi.reshape(P, 3 * K), dPsi.reshape(P, 3 * K)], axis=1)
+    return loss, dy
+
+
+def backward(theta, cache, y, t, s, dy=None):
+    """Gradient with respect to theta (flat, same order) of ``loss(y, t, s)``,
+    or, given ``dy``, of any loss whose gradient in y is dy."""
     p = unpack(theta)
     X, nbr, h1, a, c1, h3, g, h4, P = (cache[k] for k in
                                        ("X", "nbr", "h1", "a", "c1", "h3", "g", "h4", "P"))
     k = nbr.shape[1]
-    tt = max(float((t * t).sum()), 1e-300)
-    dy = 2.0 * s * (s * y - t) / tt
+    if dy is None:
+        tt = max(float((t * t).sum()), 1e-300)
+        dy = 2.0 * s * (s * y - t) / tt
     grads = {}
     grads["W5"] = dy.T @ h4
     grads["b5"] = dy.sum(axis=0)

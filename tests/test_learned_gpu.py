@@ -44,7 +44,7 @@ class _FakeCloud:
         self.h1, self.a, self.m = (torch.zeros((P, W), **e) for _ in range(3))
         self.c1 = torch.zeros((P, 2 * W), **e)
         self.h3, self.h4 = torch.zeros((P, W), **e), torch.zeros((P, W), **e)
-        self.y = torch.zeros((P, 3), **e)
+        self.y = torch.zeros((P, L.N_OUT), **e)
 
 
 def test_knn_matches_oracle():
@@ -89,7 +89,7 @@ def test_forward_backward_match_oracle():
     net = L.PointNet(theta)
     cl = _FakeCloud(300, seed=2)
     s = 0.8
-    tgt = torch.as_tensor(rng.normal(size=(cl.P, 3)).astype(np.float32), device=cl.dev)
+    tgt = torch.as_tensor(rng.normal(size=(cl.P, L.N_OUT)).astype(np.float32), device=cl.dev)
     net.forward(cl)
     dy = torch.empty_like(cl.y)
     loss = float(net.loss_grad(cl, tgt, dy, s=s).item())
@@ -119,6 +119,48 @@ def test_forward_backward_match_oracle():
     eb = _rel(cl.y.cpu().numpy(), y32.cpu().numpy())
     print(f"bf16 inference rel diff {eb:.2e}", flush=True)
     assert eb < 3e-2
+
+
+def test_lowrank_loss_and_backward_match_oracle():
+    from mbem_amd import learned as L
+    from oracle import learned as ol
+    rng = np.random.default_rng(7)
+    theta = (rng.normal(scale=0.2, size=L.n_params())).astype(np.float32)
+    net = L.PointNet(theta)
+    cl = _FakeCloud(257, seed=3)
+    B = 5
+    R = torch.as_tensor(rng.normal(size=(B, 3 * cl.P)).astype(np.float32), device=cl.dev)
+    T = torch.as_tensor(rng.normal(size=(B, 3 * cl.P)).astype(np.float32), device=cl.dev)
+    net.forward(cl)
+    dy = torch.empty_like(cl.y)
+    loss = float(net.lowrank_loss_grad(cl, R, T, dy).item())
+    net.zero_grad()
+    net.backward(cl, dy)
+    nbr = cl.nbr.cpu().numpy().astype(np.int64)
+    th = theta.astype(np.float64)
+    y_ref, cache = ol.forward(th, cl.X.cpu().numpy().astype(np.float64), nbr)
+    L_ref, dy_ref = ol.lowrank_loss(y_ref, R.cpu().numpy().astype(np.float64),
+                                    T.cpu().numpy().astype(np.float64))
+    g_ref = ol.backward(th, cache, y_ref, None, 1.0, dy=dy_ref)
+    print(f"low-rank loss {loss:.6f} vs {L_ref:.6f}; dy rel err "
+          f"{_rel(dy.cpu().numpy(), dy_ref):.2e}; grad rel err "
+          f"{_rel(net.grad.cpu().numpy(), g_ref):.2e}", flush=True)
+    assert abs(loss - L_ref) <= 1e-4 * L_ref
+    assert _rel(dy.cpu().numpy(), dy_ref) < 1e-4
+    assert _rel(net.grad.cpu().numpy(), g_ref) < 1e-3
+    # the preconditioner's FP64 apply: z += Phi (Psi^T r)
+    Y = cl.y
+    r = torch.as_tensor(rng.normal(size=3 * cl.P), dtype=torch.float64, device=cl.dev)
+    from mbem_amd._lib import call, ptr, stream
+    part = torch.zeros(8 * L.RANK, dtype=torch.float64, device=cl.dev)
+    c = torch.zeros(L.RANK, dtype=torch.float64, device=cl.dev)
+    z = torch.zeros_like(r)
+    call("mbem_lr_coef", ptr(Y), cl.P, L.RANK, 3 * L.RANK, ptr(r), 3 * cl.P, 1, 1, ptr(part), 8,
+         ptr(c), stream())
+    call("mbem_lr_expand", ptr(Y), cl.P, L.RANK, 0, ptr(c), 1, ptr(z), 3 * cl.P, 1, 1.0, 1, stream())
+    Phi, Psi = ol.bases(Y.cpu().numpy().astype(np.float64))
+    zr = Phi @ (Psi.T @ r.cpu().numpy())
+    assert _rel(z.cpu().numpy(), zr) < 1e-12
 
 
 def test_adam_matches_oracle():

@@ -17,7 +17,7 @@ def _setup(P=37, k=16, seed=0):
     X = rng.normal(size=(P, ol.N_IN))
     pts = rng.normal(size=(P, 3))
     nbr = ol.knn(pts, k)
-    t = rng.normal(size=(P, 3))
+    t = rng.normal(size=(P, ol.N_OUT))
     return theta, X, nbr, t
 
 
@@ -51,6 +51,27 @@ def test_backward_matches_finite_differences(s):
             fd = (fp - fm) / (2 * h)
             assert abs(fd - g[q]) <= 1e-6 * max(1.0, abs(fd)) + 1e-7, (name, idx, fd, g[q])
         o += size
+
+
+def test_lowrank_loss_gradient_matches_finite_differences():
+    rng = np.random.default_rng(6)
+    P, B = 11, 3
+    y = rng.normal(size=(P, ol.N_OUT))
+    R = rng.normal(size=(B, 3 * P))
+    T = rng.normal(size=(B, 3 * P))
+    L, dy = ol.lowrank_loss(y, R, T)
+    Phi, Psi = ol.bases(y)
+    # the bases' layout: row 3p + a of Phi is y[p, aK:(a+1)K]
+    K = ol.RANK
+    assert np.array_equal(Phi[3 * 4 + 2], y[4, 2 * K:3 * K])
+    assert np.array_equal(Psi[3 * 4 + 1], y[4, 3 * K + K:3 * K + 2 * K])
+    h = 1e-6
+    for idx in rng.choice(y.size, size=40, replace=False):
+        yp, ym = y.copy(), y.copy()
+        yp.flat[idx] += h
+        ym.flat[idx] -= h
+        fd = (ol.lowrank_loss(yp, R, T)[0] - ol.lowrank_loss(ym, R, T)[0]) / (2 * h)
+        assert abs(fd - dy.flat[idx]) <= 1e-6 * max(1.0, abs(fd)) + 1e-8
 
 
 def test_backward_catches_a_planted_error():
