@@ -433,6 +433,8 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=3000)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
+    ap.add_argument("--recycle", type=int, default=60,
+                    help="recycled vectors kept across the mixed method's inner solves")
     ap.add_argument("--workload", choices=("solve", "laplace", "readout"), default="solve",
                     help="solve: the N=20k dense solve (the headline metric); laplace: the "
                          "batched Laplace-state solves; readout: stress readout throughput")
@@ -533,7 +535,7 @@ def main():
         t0 = time.perf_counter()
         x, rep = solve_mixed(a, rtol=args.rtol, storage=args.storage,
                              inner_rtol=args.inner_rtol, precond_bj_chunk=args.bj_chunk_mixed,
-                             precond_grouping=args.grouping_mixed)
+                             precond_grouping=args.grouping_mixed, recycle=args.recycle)
         a.timings.update(mixed_operator_s=rep.operator_build_s,
                          precond_s=rep.precond_build_s, fgmres_s=rep.solve_s)
         return a, x, rep
@@ -588,7 +590,8 @@ def main():
             "plain_precond": f"block-Jacobi FP64, {args.grouping} grouping, chunk {args.bj_chunk}",
             "mixed_precond": (f"inner FGMRES to {args.inner_rtol:g} on the mixed operator, "
                               f"block-Jacobi FP32, {args.grouping_mixed} grouping, chunk "
-                              f"{args.bj_chunk_mixed}"),
+                              f"{args.bj_chunk_mixed}, {args.recycle} recycled vectors across "
+                              f"inner solves"),
             "learned_preconditioner": "not built yet",
         },
         "roofline": {"bound": "hbm", "achieved": gemv_bytes / gemv_ms / 1e6,

@@ -97,7 +97,7 @@ _SIGS = {
     "mbem_adam": [_p, _p, _p, _p, _i64, _f, _f, _f, _f, _f, _p],
     "mbem_hgemm": [_p, _i64, _p, _i64, _i64, _i64, _p, _int, _p, _i64, _p],
     "mbem_lr_coef": [_p, _i64, _i64, _i64, _p, _i64, _i64, _int, _p, _i64, _p, _p],
-    "mbem_lr_expand": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _int, _d, _int, _p],
+    "mbem_lr_expand": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _int, _d, _int, _p, _p],
     "mbem_lr_outer": [_p, _i64, _p, _i64, _i64, _i64, _i64, _p, _p],
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }

@@ -436,12 +436,14 @@ __global__ __launch_bounds__(256) void k_lr_coef_sum(const double* __restrict__ 
     C[q] = s;
 }
 
-// Out[b, 3p + a] (+)= alpha sum_k Y[p, off + a K + k] C[b, k]
+// Out[b, 3p + a] (+)= alpha d[3p + a] sum_k Y[p, off + a K + k] C[b, k]
+// (d = 1 when no row scale is given)
 template <typename T>
 __global__ __launch_bounds__(256) void k_lr_expand(const float* __restrict__ Y, long P, long K,
                                                    long off, const double* __restrict__ C, long B,
                                                    T* __restrict__ Out, long ldo, double alpha,
-                                                   int accumulate)
+                                                   int accumulate,
+                                                   const double* __restrict__ rowscale)
 {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= B * 3 * P) return;
@@ -451,6 +453,7 @@ __global__ __launch_bounds__(256) void k_lr_expand(const float* __restrict__ Y, 
     const double* c = C + b * K;
     double s = 0.0;
     for (long k = 0; k < K; ++k) s += (double)y[k] * c[k];
+    if (rowscale) s *= rowscale[i];
     T* o = Out + b * ldo + i;
     *o = accumulate ? (T)((double)*o + alpha * s) : (T)(alpha * s);
 }
@@ -653,17 +656,17 @@ int mbem_lr_coef(const float* Y, int64_t P, int64_t K, int64_t off, const void* 
 
 int mbem_lr_expand(const float* Y, int64_t P, int64_t K, int64_t off, const double* C, int64_t B,
                    void* Out, int64_t ldo, int out_fp64, double alpha, int accumulate,
-                   hipStream_t stream)
+                   const double* rowscale, hipStream_t stream)
 {
     if (P < 1 || K < 1 || B < 1 || !Y || !C || !Out) return MBEM_EINVAL;
     if (off != 0 && off != 3 * K) return MBEM_EINVAL;
     const long n = B * 3 * P;
     if (out_fp64)
         hipLaunchKernelGGL(k_lr_expand<double>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, Y,
-                           P, K, off, C, B, (double*)Out, ldo, alpha, accumulate);
+                           P, K, off, C, B, (double*)Out, ldo, alpha, accumulate, rowscale);
     else
         hipLaunchKernelGGL(k_lr_expand<float>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, Y,
-                           P, K, off, C, B, (float*)Out, ldo, alpha, accumulate);
+                           P, K, off, C, B, (float*)Out, ldo, alpha, accumulate, rowscale);
     return launch_status();
 }
 

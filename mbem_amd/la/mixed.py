@@ -26,7 +26,7 @@ import torch
 from .. import _lib, defaults
 from .._lib import call, ptr, stream
 from ..kernels import basis as kb
-from .solver import SolveReport, fgmres
+from .solver import SolveReport, _Krylov, _solve_upper, axpby, fgmres
 
 F64 = torch.float64
 I64 = torch.int64
@@ -163,6 +163,127 @@ class _InnerSolve:
         return z
 
 
+class _RecycledInner:
+    """Inner solve with a recycled subspace (GCRO with a fixed recycle space;
+    the reference's GCRO-DR is solver.py:248-395).  The inner operator is the
+    same at every outer step, so each inner solve keeps its Krylov space as
+    pairs (U, C) with op U = C and C^T C = I; the next solve first takes the
+    part of its residual in span C (x0 = U C^T r) and then runs flexible
+    GMRES on (I - C C^T) op, minimising over span U + the new Krylov space.
+    After the solve, op (Z - U Hc) = V Hbar (Hbar unrotated) gives the new
+    pairs: Hbar = Q R, C_new = V Q, U_new = (Z - U Hc) R^-1, appended until
+    ``kmax`` vectors are held."""
+
+    def __init__(self, op, M, rtol, maxiter, n, dev, kmax=60):
+        self.op, self.M = op, M
+        self.rtol, self.maxiter = rtol, maxiter
+        self.kmax = int(kmax)
+        self.U = torch.zeros((self.kmax, n), dtype=F64, device=dev)
+        self.C = torch.zeros((self.kmax, n), dtype=F64, device=dev)
+        self.k = 0
+        self.ws = _Krylov(n, maxiter, dev)
+        self.hc = torch.zeros(max(self.kmax, 1), dtype=F64, device=dev)
+        self.cwork = torch.empty(int(_lib.load().mbem_multidot_work(max(self.kmax, 1), n)),
+                                 dtype=F64, device=dev)
+        self.iterations = []
+        self.n = n
+
+    def apply_into(self, r, z):
+        ws, n, k = self.ws, self.n, self.k
+        m = self.maxiter
+        x = z
+        x.zero_()
+        rr = r.clone()
+        r_norm = ws.norm(rr)
+        if r_norm == 0.0:
+            self.iterations.append(0)
+            return z
+        if k > 0:
+            coef = torch.empty(k, dtype=F64, device=rr.device)
+            call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(rr), ptr(coef),
+                 ptr(self.cwork), stream())
+            ws.axpy_rows(self.U, k, coef, x, 1.0)
+            ws.axpy_rows(self.C, k, coef, rr, -1.0)
+        beta = ws.norm(rr)
+        V, Z = ws.V, ws.Z
+        H = np.zeros((m + 1, m))
+        Hraw = np.zeros((m + 1, m))
+        Hc = np.zeros((max(k, 1), m))
+        cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
+        j_done = 0
+        if beta / r_norm > self.rtol:
+            axpby(1.0 / beta, rr, 0.0, V[0])
+            g[0] = beta
+            for j in range(m):
+                self.M.apply_into(V[j], Z[j])
+                w = V[j + 1]
+                self.op.apply_into(Z[j], w)
+                if k > 0:
+                    call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(w),
+                         ptr(self.hc), ptr(self.cwork), stream())
+                    ws.axpy_rows(self.C, k, self.hc, w, -1.0)
+                hsum, wnorm = ws.cgs2(V, j + 1, w)
+                if k > 0:
+                    Hc[:k, j] = self.hc[:k].cpu().numpy()
+                H[:j + 1, j] = hsum
+                H[j + 1, j] = wnorm
+                Hraw[:j + 2, j] = H[:j + 2, j]
+                if wnorm > 1e-300:
+                    axpby(1.0 / wnorm, w, 0.0, w)
+                for i in range(j):
+                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                    H[i, j] = t
+                d = np.hypot(H[j, j], H[j + 1, j])
+                j_done = j + 1
+                if d == 0.0:
+                    break
+                cs[j], sn[j] = H[j, j] / d, H[j + 1, j] / d
+                H[j, j] = d
+                H[j + 1, j] = 0.0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j] * g[j]
+                if abs(g[j + 1]) / r_norm < self.rtol or wnorm <= 1e-300:
+                    break
+        self.iterations.append(j_done)
+        if j_done == 0:
+            return z
+        y = _solve_upper(H[:j_done, :j_done], g[:j_done])
+        yd = torch.as_tensor(y, dtype=F64, device=x.device)
+        ws.axpy_rows(Z, j_done, yd, x, 1.0)
+        if k > 0:
+            cy = torch.as_tensor(Hc[:k, :j_done] @ y, dtype=F64, device=x.device)
+            ws.axpy_rows(self.U, k, cy, x, -1.0)
+        self._grow(j_done, Hraw[:j_done + 1, :j_done], Hc[:k, :j_done] if k > 0 else None)
+        return z
+
+    def _grow(self, j, Hbar, Hc):
+        """Append the pairs of this solve's Krylov space to (U, C)."""
+        add = min(j, self.kmax - self.k)
+        if add <= 0:
+            return
+        Q, R = np.linalg.qr(Hbar)                  # (j+1) x j, j x j
+        if np.min(np.abs(np.diag(R))) < 1e-12 * np.max(np.abs(np.diag(R))):
+            return
+        Rinv = np.linalg.inv(R)
+        ws, k, dev = self.ws, self.k, self.U.device
+        V, Z = ws.V, ws.Z
+        W = torch.empty((j, self.n), dtype=F64, device=dev)
+        W.copy_(Z[:j])
+        if Hc is not None and k > 0:
+            for i in range(j):
+                ws.axpy_rows(self.U, k, torch.as_tensor(Hc[:, i], dtype=F64, device=dev), W[i],
+                             -1.0)
+        for i in range(add):
+            self.C[k + i].zero_()
+            ws.axpy_rows(V, j + 1, torch.as_tensor(Q[:, i], dtype=F64, device=dev), self.C[k + i],
+                         1.0)
+            self.U[k + i].zero_()
+            ws.axpy_rows(W, j, torch.as_tensor(Rinv[:, i], dtype=F64, device=dev), self.U[k + i],
+                         1.0)
+        self.k = k + add
+
+
 @dataclass
 class MixedReport:
     outer: SolveReport
@@ -190,8 +311,10 @@ class MixedReport:
 def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
                 inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
                 outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=3000,
-                precond_grouping: str = "region", pattern=None):
-    """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport)."""
+                precond_grouping: str = "region", pattern=None, recycle: int = 60):
+    """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport).
+    ``recycle`` > 0 keeps up to that many recycled vectors across the inner
+    solves (``_RecycledInner``)."""
     from .preconditioner import BlockGaussSeidel
     t0 = time.perf_counter()
     op = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pattern)
@@ -200,7 +323,11 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
                          grouping=precond_grouping, inverse_dtype="f32", coupling=op)
     torch.cuda.current_stream().synchronize()
     t2 = time.perf_counter()
-    inner = _InnerSolve(op, M, inner_rtol, inner_maxiter)
+    if recycle:
+        inner = _RecycledInner(op, M, inner_rtol, inner_maxiter, op.n, assembled.A_dev.device,
+                               kmax=recycle)
+    else:
+        inner = _InnerSolve(op, M, inner_rtol, inner_maxiter)
     x, outer = fgmres(assembled, assembled.b_dev, M=inner, rtol=rtol, restart=outer_restart,
                       maxiter=outer_maxiter)
     torch.cuda.current_stream().synchronize()

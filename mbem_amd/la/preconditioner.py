@@ -254,6 +254,19 @@ class BlockGaussSeidel:
         z = torch.empty_like(r)
         return self.apply_into(r, z)
 
+    def inverse_diagonal(self) -> torch.Tensor:
+        """diag(M) as an FP64 device vector over the unknowns: the diagonal of
+        each chunk inverse scattered to its DOFs."""
+        n = self.asm.layout.n_unknowns
+        d = torch.zeros(n, dtype=F64, device=self.inv.device)
+        cptr = self.chunk_ptr.cpu().numpy()
+        iptr = self.inv_ptr.cpu().numpy()
+        for c in range(len(cptr) - 1):
+            m = int(cptr[c + 1] - cptr[c])
+            M = self.inv[int(iptr[c]):int(iptr[c]) + m * m].view(m, m)
+            d[self.idx[int(cptr[c]):int(cptr[c + 1])]] = M.diagonal().to(F64)
+        return d
+
     def summary(self) -> dict:
         """preconditioner.py:502-519."""
         return {

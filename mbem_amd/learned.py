@@ -1,7 +1,10 @@
 """Learned FGMRES preconditioner: a point-cloud network over the boundary
 elements that proposes a low-rank correction of block-Jacobi.
 
-    M(r) = BJ(r) + Phi (Psi^T r),   Phi, Psi: n x K  (K = 16)
+    M(r) = BJ(r) + D Phi (Psi^T r) / n,   Phi, Psi: n x K  (K = 16)
+
+with D the diagonal of the block-Jacobi inverse (so Phi is dimensionless
+whatever the slot kind and the moduli).
 
 One point per boundary triangle (its three unknowns are one slot triple).
 The network sees geometry only — per point the centroid (scaled to the unit
@@ -17,9 +20,9 @@ of Phi and Psi at the point's three unknowns; ReLU between.
 Training: residual/correction pairs from our own solves.  For a system
 A x = b, restarted FGMRES cycles with block-Jacobi give iterates x_c; with
 the converged x*, r_c = b - A x_c and e_c = x* - x_c satisfy A e_c = r_c.  The
-network learns the part of e_c that block-Jacobi misses: with r^ = r / |r|
-and t^ = (e - BJ(r)) / |r|, the loss over a geometry's pairs is
-(1/B) sum_b |Phi Psi^T r^_b - t^_b|^2 / |t^_b|^2.  Features, forward,
+network learns the part of e_c that block-Jacobi misses: with s = RMS(r),
+r^ = r / (s n) and t^ = (e - BJ(r)) / (s D), the loss over a geometry's
+pairs is (1/B) sum_b |Phi Psi^T r^_b - t^_b|^2 / |t^_b|^2.  Features, forward,
 backward and Adam run in the HIP kernels of csrc/learned.hip (FP32);
 inference runs the dense layers on the BF16 matrix cores (``mbem_hgemm``).
 The network has no counterpart in the reference; block-Jacobi FGMRES
@@ -67,7 +70,7 @@ def init_params(seed: int = 0) -> np.ndarray:
         if name.startswith("b"):
             out.append(np.zeros(shp))
         else:
-            lim = math.sqrt(6.0 / shp[1]) * (0.05 if name == "W5" else 1.0)
+            lim = math.sqrt(6.0 / shp[1]) * (0.1 if name == "W5" else 1.0)
             out.append(rng.uniform(-lim, lim, size=shp))
     return np.concatenate([a.reshape(-1) for a in out]).astype(np.float32)
 
@@ -318,7 +321,7 @@ class PointNet:
         dpred = torch.empty((B, n), dtype=F32, device=self.dev)
         lb = torch.empty(B, dtype=F32, device=self.dev)
         call("mbem_lr_coef", ptr(cl.y), P, K, 3 * K, ptr(R), n, B, 0, ptr(part), ns, ptr(C), stream())
-        call("mbem_lr_expand", ptr(cl.y), P, K, 0, ptr(C), B, ptr(pred), n, 0, 1.0, 0, stream())
+        call("mbem_lr_expand", ptr(cl.y), P, K, 0, ptr(C), B, ptr(pred), n, 0, 1.0, 0, None, stream())
         call("mbem_lr_loss", ptr(pred), ptr(T), n, B, ptr(dpred), ptr(lb), stream())
         call("mbem_lr_outer", ptr(dpred), n, ptr(C), B, P, K, 0, ptr(dy), stream())
         call("mbem_lr_coef", ptr(cl.y), P, K, 0, ptr(dpred), n, B, 0, ptr(part), ns, ptr(dC),
@@ -345,10 +348,10 @@ class PointNet:
 
 # ---------------------------------------------------------------------------
 class LearnedPreconditioner:
-    """z = BJ(r) + Phi (Psi^T r): a linear right preconditioner for FGMRES
-    (``apply_into(r, z)``).  The network runs once, on the BF16 matrix cores
-    (or FP32), when the preconditioner is made; each application adds two
-    n x K products to the block-Jacobi apply."""
+    """z = BJ(r) + D Phi (Psi^T r) / n: a linear right preconditioner for
+    FGMRES (``apply_into(r, z)``).  The network runs once, on the BF16 matrix
+    cores (or FP32), when the preconditioner is made; each application adds
+    two n x K products to the block-Jacobi apply."""
 
     def __init__(self, base, layout, net: PointNet = None, bf16: bool = True):
         self.base = base
@@ -358,6 +361,7 @@ class LearnedPreconditioner:
         t0 = time.perf_counter()
         self.net.forward(self.cloud, bf16=bf16)
         self.Y = self.cloud.y.clone()
+        self.D = base.inverse_diagonal().abs().clamp_min(1e-300)
         torch.cuda.current_stream().synchronize()
         self.infer_s = time.perf_counter() - t0
         P = self.cloud.P
@@ -371,8 +375,8 @@ class LearnedPreconditioner:
         P, K = self.cloud.P, RANK
         call("mbem_lr_coef", ptr(self.Y), P, K, 3 * K, ptr(r), 3 * P, 1, 1, ptr(self._part), self.ns,
              ptr(self._c), stream())
-        call("mbem_lr_expand", ptr(self.Y), P, K, 0, ptr(self._c), 1, ptr(z), 3 * P, 1, 1.0, 1,
-             stream())
+        call("mbem_lr_expand", ptr(self.Y), P, K, 0, ptr(self._c), 1, ptr(z), 3 * P, 1, 1.0 / (3 * P),
+             1, ptr(self.D), stream())
         self.applies += 1
         return z
 
@@ -416,23 +420,27 @@ def residual_pairs(asm, M, cycles: int = 8, restart: int = 10, rtol: float = 1e-
 
 
 def make_samples(asm, M, cloud: Cloud, pairs):
-    """One batch Sample of a geometry's (r, e) pairs (uses block-Jacobi M)."""
+    """One batch Sample of a geometry's (r, e) pairs (uses block-Jacobi M):
+    r^ = r / (RMS(r) n), t^ = (e - BJ(r)) / (RMS(r) D)."""
     R, T = [], []
+    D = M.inverse_diagonal().abs().clamp_min(1e-300)
+    n = pairs[0][0].numel() if pairs else 0
     for r, e in pairs:
-        nr = float(torch.linalg.vector_norm(r))
-        if nr == 0.0:
+        rms = float(torch.linalg.vector_norm(r)) / math.sqrt(n)
+        if rms == 0.0:
             continue
         z = torch.empty_like(r)
         M.apply_into(r, z)
-        R.append((r / nr).float())
-        T.append(((e - z) / nr).float())
+        R.append((r / (rms * n)).float())
+        T.append(((e - z) / (rms * D)).float())
     return [Sample(cloud, torch.stack(R).contiguous(), torch.stack(T).contiguous())]
 
 
 def train(net: PointNet, samples, epochs: int = 10, lr: float = 1e-3, seed: int = 0,
-          log=print, lr_decay: float = 0.7):
-    """Adam over the samples, one step per sample, shuffled per epoch; returns
-    the mean loss per epoch."""
+          log=print, lr_final: float = 0.1):
+    """Adam over the samples (one step per geometry batch, shuffled per
+    epoch), rate on a cosine from lr to lr * lr_final; returns the mean loss
+    per epoch."""
     rng = np.random.default_rng(seed)
     hist = []
     for ep in range(epochs):
@@ -448,7 +456,8 @@ def train(net: PointNet, samples, epochs: int = 10, lr: float = 1e-3, seed: int 
             tot += loss
             net.zero_grad()
             net.backward(cl, dy)
-            net.adam(lr=lr * (lr_decay ** ep))
+            frac = ep / max(epochs - 1, 1)
+            net.adam(lr=lr * (lr_final + (1 - lr_final) * 0.5 * (1 + math.cos(math.pi * frac))))
         mean = float(tot.item()) / max(len(samples), 1)
         hist.append(mean)
         log(f"[learned] epoch {ep}: mean loss {mean:.4f} ({time.perf_counter() - t0:.1f} s)")

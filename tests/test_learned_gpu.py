@@ -157,9 +157,11 @@ def test_lowrank_loss_and_backward_match_oracle():
     z = torch.zeros_like(r)
     call("mbem_lr_coef", ptr(Y), cl.P, L.RANK, 3 * L.RANK, ptr(r), 3 * cl.P, 1, 1, ptr(part), 8,
          ptr(c), stream())
-    call("mbem_lr_expand", ptr(Y), cl.P, L.RANK, 0, ptr(c), 1, ptr(z), 3 * cl.P, 1, 1.0, 1, stream())
+    d = torch.as_tensor(rng.uniform(0.5, 2.0, size=3 * cl.P), dtype=torch.float64, device=cl.dev)
+    call("mbem_lr_expand", ptr(Y), cl.P, L.RANK, 0, ptr(c), 1, ptr(z), 3 * cl.P, 1, 0.25, 1, ptr(d),
+         stream())
     Phi, Psi = ol.bases(Y.cpu().numpy().astype(np.float64))
-    zr = Phi @ (Psi.T @ r.cpu().numpy())
+    zr = 0.25 * d.cpu().numpy() * (Phi @ (Psi.T @ r.cpu().numpy()))
     assert _rel(z.cpu().numpy(), zr) < 1e-12
 
 
