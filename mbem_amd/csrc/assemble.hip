@@ -81,6 +81,10 @@ __global__ __launch_bounds__(256) void k_prepare_src(const double* __restrict__ 
 // out[(3f+i)*ld + 3s+j] = scale * K_ij + beta * out (STORE); rowsum[9f+ij] +=
 // rs_scale * sum_s K_ij (ROWSUM, the calibration row sum of
 // backends/dense.py:497-499 / model/equations.py:512-537, fused).
+// One wave per R collocation rows, lane = source.  R = 1 is used: measured at
+// N = 20,000 the assembly takes 35 ms at R = 1 (2-3 waves per SIMD, no AGPR
+// spill) against 44 ms at R = 4 (occupancy 1), the source record's reuse
+// across rows being worth less than the latency hiding.
 template <int KIND, int R, bool STORE, bool ROWSUM>
 __global__ __launch_bounds__(256) void k_block(const double* __restrict__ xf, long nf,
                                                const double* __restrict__ src, long lds, long ns,
@@ -408,7 +412,7 @@ int mbem_u_block(const double* x_field, int64_t nf, const double* src, int64_t l
     if (nf == 0 || ns == 0) return MBEM_OK;
     if (!x_field || !out) return MBEM_EINVAL;
     Coef cf{{g[0], g[1], g[2], 0.0, 0.0, 0.0}};
-    return launch_block<0, 4>(x_field, nf, src, ld_src, ns, cf, out, ld_out, scale, beta, nullptr, 0.0, 1, stream);
+    return launch_block<0, 1>(x_field, nf, src, ld_src, ns, cf, out, ld_out, scale, beta, nullptr, 0.0, 1, stream);
 }
 
 int mbem_t_block(const double* x_field, int64_t nf, const double* src, int64_t ld_src, int64_t ns,
@@ -420,7 +424,7 @@ int mbem_t_block(const double* x_field, int64_t nf, const double* src, int64_t l
     if (nf == 0 || ns == 0) return MBEM_OK;
     if (!x_field || (store && !out) || (!store && !rowsum)) return MBEM_EINVAL;
     Coef cf{{c[0], c[1], c[2], c[3], c[4], c[5]}};
-    return launch_block<1, 4>(x_field, nf, src, ld_src, ns, cf, out, ld_out, scale, beta, rowsum, rs_scale, store, stream);
+    return launch_block<1, 1>(x_field, nf, src, ld_src, ns, cf, out, ld_out, scale, beta, rowsum, rs_scale, store, stream);
 }
 
 int mbem_u_basis(const double* x_field, int64_t nf, const double* src, int64_t ld_src, int64_t ns,
