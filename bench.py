@@ -214,8 +214,11 @@ def bench_laplace(args):
     K = len(states) * len(slips)
     K += K & 1
     g = torch.full((K,), 1.5, dtype=torch.float64, device="cuda")
-    op = GOperator(ls.A[0], ls.A[1], g)
+    op = GOperator(ls.A[0], ls.A[1], g, pair=(ls.AA, ls.L))
+    op._prepare()
     op.cfg = rep.gemm_cfg["gemm_g"]
+    from mbem_amd.la import autotune
+    tuned = autotune.report(f"gemm_g n={n} K={K} lib=True")
     X = torch.randn(n, K, dtype=torch.float64, device="cuda")
     Y = torch.empty_like(X)
     gemm_ms = _events_time(lambda: op.apply(X, Y), 5)
@@ -266,6 +269,8 @@ def bench_laplace(args):
             "lockstep_steps": rep.batched.steps, "cycles": rep.batched.cycles,
             "max_relres": float(rep.batched.relres.max()),
             "gemm_cfg": rep.gemm_cfg,
+            "gemm_candidates_ms": tuned["ms"] if tuned else None,
+            "gemm_library_id": op.LIB,
             "sequential_states_timed": len(pick),
             "sequential_per_state_s": [float(v) for v in seq],
             "sequential_estimate_s": seq_est,
@@ -275,7 +280,9 @@ def bench_laplace(args):
         "roofline": {"bound": "mfma", "achieved": flops / gemm_ms / 1e9,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": flops / gemm_ms / 1e9 / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": "k_mgemm (A0 X + A1 X diag(g), FP64 MFMA)",
+                     "kernel": ("library DGEMM [A0 | A1][X ; X g] (rocBLAS, FP64 MFMA)"
+                                if op.cfg == op.LIB else
+                                "k_mgemm (A0 X + A1 X diag(g), FP64 MFMA)"),
                      "algorithmic_flops": flops, "avg_ms": gemm_ms},
         "cpu_baseline": None,
     }

@@ -226,6 +226,13 @@ int mbem_gemm_g(const double *A0, const double *A1, int64_t lda, int64_t m, int6
                 const double *X, int64_t ldx, int64_t ncols, const double *g, double *Y,
                 int64_t ldy, double alpha, double beta, int cfg, hipStream_t stream);
 
+/* Row-major C (m x n) = alpha A (m x k) B (k x n) + beta C through the
+ * library DGEMM (rocBLAS, matrix cores): the autotuner's library candidate for
+ * the batched operator, applied to [A0 | A1] [X ; X diag(g)]. */
+int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                        const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+                        hipStream_t stream);
+
 /* Grouped GEMM over a device work table of nwork items (mbem_work_item_bytes
  * each: chunk inverse offset and leading dimension, chunk size, block offset,
  * column count, tile row, tile column): Y[rows of chunk, cols] = Inv X[...].
@@ -336,13 +343,14 @@ int mbem_hgemm(const float *A, int64_t lda, const float *W, int64_t K, int64_t N
 /* Low-rank correction from the network output Y (P x 6K): Phi = Y[:, 0:3K],
  * Psi = Y[:, 3K:6K] as 3P x K bases (row 3p + a = Y[p, a K + off ..]).
  * C (B x K) = Bas^T V_b (V FP64 or FP32, rows ldv apart; part: nslices x B x K
- * doubles); Out_b (+)= alpha Bas C_b; dY[:, off..] = sum_b U_b C_b^T; batch
+ * doubles); Out_b (+)= alpha diag(rowscale) Bas C_b (rowscale optional, FP64
+ * per unknown); dY[:, off..] = sum_b U_b C_b^T; batch
  * loss (1/B) sum_b |pred_b - t_b|^2 / |t_b|^2 with its gradient. */
 int mbem_lr_coef(const float *Y, int64_t P, int64_t K, int64_t off, const void *V, int64_t ldv,
                  int64_t B, int v_fp64, double *part, int64_t nslices, double *C, hipStream_t stream);
 int mbem_lr_expand(const float *Y, int64_t P, int64_t K, int64_t off, const double *C, int64_t B,
                    void *Out, int64_t ldo, int out_fp64, double alpha, int accumulate,
-                   hipStream_t stream);
+                   const double *rowscale, hipStream_t stream);
 int mbem_lr_outer(const float *U, int64_t ldu, const double *C, int64_t B, int64_t P, int64_t K,
                   int64_t off, float *dY, hipStream_t stream);
 int mbem_lr_loss(const float *pred, const float *tgt, int64_t n, int64_t B, float *dpred,

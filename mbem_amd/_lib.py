@@ -65,6 +65,7 @@ _SIGS = {
     "mbem_mgemm_config": [_int, _p],
     "mbem_gemm_g": [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _int, _p],
     "mbem_work_item_bytes": [],
+    "mbem_dgemm_rowmajor": [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p],
     "mbem_gemm_grouped": [_p, _p, _i64, _p, _i64, _p, _i64, _int, _p],
     "mbem_bdot": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p],
     "mbem_breduce": [_p, _i64, _i64, _i64, _p, _i64, _i64, _int, _p, _p],

@@ -20,11 +20,14 @@
 //
 // Block vectors are row-major n x K (ld = K): column c is system c.
 #include <hip/hip_runtime.h>
+#include <rocblas/rocblas.h>
 #include <stdint.h>
 
 #include "../../include/mbem_hip.h"
 
 namespace {
+
+rocblas_handle g_blas = nullptr;
 
 typedef double dbl2 __attribute__((ext_vector_type(2)));
 typedef double dbl4 __attribute__((ext_vector_type(4)));
@@ -500,6 +503,25 @@ int mbem_gemm_grouped(const double* inv, const double* X, int64_t ldx, double* Y
 }
 
 int mbem_work_item_bytes(void) { return (int)sizeof(Work); }
+
+int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                        const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                        hipStream_t stream)
+{
+    if (m < 0 || n < 0 || k < 0 || (m > 0 && n > 0 && (!A || !B || !C)) || lda < k || ldb < n || ldc < n)
+        return MBEM_EINVAL;
+    if (m == 0 || n == 0) return MBEM_OK;
+    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX || lda > INT32_MAX || ldb > INT32_MAX || ldc > INT32_MAX)
+        return MBEM_EINVAL;
+    if (!g_blas && rocblas_create_handle(&g_blas) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_blas, stream) != rocblas_status_success) return MBEM_EHIP;
+    // row-major C (m x n) = A (m x k) B (k x n)  <=>  column-major C^T = B^T A^T
+    const rocblas_status st = rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none,
+                                            (rocblas_int)n, (rocblas_int)m, (rocblas_int)k, &alpha, B,
+                                            (rocblas_int)ldb, A, (rocblas_int)lda, &beta, C,
+                                            (rocblas_int)ldc);
+    return st == rocblas_status_success ? launch_status() : MBEM_EHIP;
+}
 
 int mbem_bdot(const double* V, int64_t bstride, int64_t cnt, const double* W, int64_t n, int64_t K,
               double* partial, int64_t nslots, hipStream_t stream)

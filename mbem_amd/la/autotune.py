@@ -61,7 +61,9 @@ def _save_file():
 def tune(key: str, run, candidates=None, reps: int = 3, warmup: int = 1) -> int:
     """Index of the fastest configuration for ``key``; ``run(cfg)`` launches
     the GEMM once with configuration ``cfg`` on the current stream (it must
-    not allocate or synchronise).  Timings are kept in ``report(key)``."""
+    not allocate or synchronise).  ``candidates`` defaults to the compiled
+    tiles; a caller may add its own ids (the library GEMM).  Timings are kept
+    in ``report(key)``."""
     _load_file()
     if key in _CACHE:
         return int(_CACHE[key]["best"])

@@ -41,9 +41,15 @@ I64 = torch.int64
 
 
 class GOperator:
-    """Y = alpha B(g) X + beta Y for n x K blocks, B(g_c) = A0 + g_c A1."""
+    """Y = alpha B(g) X + beta Y for n x K blocks, B(g_c) = A0 + g_c A1.
 
-    def __init__(self, A0: torch.Tensor, A1: torch.Tensor, g: torch.Tensor, tune: bool = True):
+    Candidates: the compiled tiles of the project's matrix-core kernel
+    (``mbem_gemm_g``) and, when A0 and A1 sit side by side in one n x 2L
+    row-major buffer (``pair=(AA, L)``: A0 = AA[:, :n], A1 = AA[:, L:L+n]),
+    the library DGEMM on [A0 | A1] [X ; X diag(g)] (``LIB``).  ``tune``
+    times them on the real shape and keeps the fastest."""
+
+    def __init__(self, A0: torch.Tensor, A1: torch.Tensor, g: torch.Tensor, pair=None):
         n = A0.shape[0]
         if A0.shape != (n, n) or A1.shape != (n, n) or A0.stride(0) != A1.stride(0):
             raise ValueError("A0 and A1 must be n x n with one leading dimension")
@@ -53,15 +59,42 @@ class GOperator:
         self.g = g.contiguous()
         self.K = self.g.numel()
         self.cfg = 0
-        self._tune = tune
+        self.LIB = len(autotune.configs())
+        self.pair = pair
+        self.X2 = None
+        if pair is not None:
+            AA, L = pair
+            if AA.shape != (n, 2 * L) or AA.data_ptr() != A0.data_ptr() or \
+                    A1.data_ptr() != AA[:, L:].data_ptr():
+                raise ValueError("pair must be the n x 2L buffer holding A0 and A1")
+
+    def candidates(self):
+        c = [i for i, _ in autotune.configs()]
+        return c + [self.LIB] if self.pair is not None else c
 
     def tune(self, X, Y):
-        """Pick the tile configuration for this shape (times every candidate)."""
-        key = f"gemm_g n={self.n} K={self.K}"
-        self.cfg = autotune.tune(key, lambda cfg: self._launch(X, Y, 1.0, 0.0, cfg))
+        """Pick the configuration for this shape (times every candidate)."""
+        key = f"gemm_g n={self.n} K={self.K} lib={self.pair is not None}"
+        self._prepare()
+        self.cfg = autotune.tune(key, lambda cfg: self._launch(X, Y, 1.0, 0.0, cfg),
+                                 candidates=self.candidates())
         return self.cfg
 
+    def _prepare(self):
+        """The library path's [X ; X g] buffer (made before any capture)."""
+        if self.pair is not None and self.X2 is None:
+            L = self.pair[1]
+            self.X2 = torch.zeros((2 * L, self.K), dtype=F64, device=self.A0.device)
+
     def _launch(self, X, Y, alpha, beta, cfg):
+        if cfg == self.LIB:
+            AA, L = self.pair
+            n, K = self.n, self.K
+            call("mbem_bcopy", ptr(X), ptr(self.X2), n * K, stream())
+            call("mbem_bscale", ptr(X), ptr(self.g), ptr(self.X2[L:]), n, K, stream())
+            call("mbem_dgemm_rowmajor", n, K, 2 * L, float(alpha), ptr(AA), AA.stride(0),
+                 ptr(self.X2), K, float(beta), ptr(Y), Y.stride(0), stream())
+            return
         call("mbem_gemm_g", ptr(self.A0), ptr(self.A1), self.A0.stride(0), self.n, self.n, ptr(X),
              X.stride(0), self.K, ptr(self.g), ptr(Y), Y.stride(0), float(alpha), float(beta),
              int(cfg), stream())
@@ -319,6 +352,7 @@ class BatchedFGMRES:
         t0 = time.perf_counter()
         st = torch.cuda.current_stream()
         self.M._table(self.M.cfg)           # device work table, made before any capture
+        self.op._prepare()
         self.B.copy_(B)
         bnorm = torch.linalg.vector_norm(self.B, dim=0).cpu().numpy()
         self.X.zero_()

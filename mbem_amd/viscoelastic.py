@@ -133,14 +133,18 @@ class LaplaceStates:
         self.device = dev
         P = torch.as_tensor(self.perm, dtype=I64, device=dev)
         self.perm_dev = P
-        ld = self.n + (self.n & 1)
+        # A0 and A1 side by side in one n x 2L buffer (L = n rounded up to even):
+        # the batched operator is then also one library GEMM [A0 | A1] [X ; X g]
+        L = self.n + (self.n & 1)
+        self.L = L
+        self.AA = torch.zeros((self.n, 2 * L), dtype=F64, device=dev)
         self.A = []
-        for part in self.parts:
-            Ap = torch.empty((self.n, ld), dtype=F64, device=dev)
-            call("mbem_permute", ptr(part.A_dev), part.A_dev.stride(0), ptr(P), self.n, ptr(Ap),
-                 ld, stream())
+        for k, part in enumerate(self.parts):
+            view = self.AA[:, k * L:k * L + self.n]
+            call("mbem_permute", ptr(part.A_dev), part.A_dev.stride(0), ptr(P), self.n, ptr(view),
+                 2 * L, stream())
             part.A_dev = None
-            self.A.append(Ap[:, :self.n])
+            self.A.append(view)
         torch.cuda.synchronize()
         torch.cuda.empty_cache()
         self.permute_s = time.perf_counter() - t0
@@ -238,7 +242,7 @@ class LaplaceStates:
         Bp = Bmat[self.perm_dev].contiguous()
         rep.rhs_s = time.perf_counter() - t0
         g_dev = torch.as_tensor(gcol, dtype=F64, device=self.device)
-        op = GOperator(self.A[0], self.A[1], g_dev)
+        op = GOperator(self.A[0], self.A[1], g_dev, pair=(self.AA, self.L))
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
@@ -273,5 +277,6 @@ class LaplaceStates:
 
     def release(self):
         self.A = []
+        self.AA = None
         for p in self.parts:
             p.release()

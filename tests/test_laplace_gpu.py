@@ -110,18 +110,49 @@ def test_grouped_block_jacobi_every_config():
         assert err < 1e-12, (cfg, err)
 
 
-def test_graph_replays_match_eager():
+def _pair(n, seed, dev):
+    """A0, A1 side by side in one n x 2L buffer, as LaplaceStates keeps them."""
+    torch.manual_seed(seed)
+    L = n + (n & 1)
+    AA = torch.zeros(n, 2 * L, dtype=torch.float64, device=dev)
+    A0, A1 = AA[:, :n], AA[:, L:L + n]
+    A0.copy_(torch.randn(n, n, dtype=torch.float64, device=dev) / math.sqrt(n)
+             + 3.0 * torch.eye(n, dtype=torch.float64, device=dev))
+    A1.copy_(torch.randn(n, n, dtype=torch.float64, device=dev) / math.sqrt(n))
+    return AA, L, A0, A1
+
+
+def test_library_candidate_matches_kernel():
+    from mbem_amd.la.batched import GOperator
+    dev = torch.device("cuda")
+    n, K = 301, 38
+    AA, L, A0, A1 = _pair(n, 11, dev)
+    g = torch.rand(K, dtype=torch.float64, device=dev) + 1.0
+    X = torch.randn(n, K, dtype=torch.float64, device=dev)
+    op = GOperator(A0, A1, g, pair=(AA, L))
+    op._prepare()
+    Y0 = torch.randn(n, K, dtype=torch.float64, device=dev)
+    ref = -0.5 * _gemm_ref(A0, A1, X, g) + 2.0 * Y0
+    for cfg in (0, op.LIB):
+        op.cfg = cfg
+        Y = Y0.clone()
+        op.apply(X, Y, alpha=-0.5, beta=2.0)
+        assert float((Y - ref).abs().max() / ref.abs().max()) < 1e-13, cfg
+
+
+@pytest.mark.parametrize("use_lib", [False, True])
+def test_graph_replays_match_eager(use_lib):
     from mbem_amd.la.batched import BatchedFGMRES, GOperator, GroupedBlockJacobi
-    torch.manual_seed(3)
     dev = torch.device("cuda")
     n, K = 240, 6
-    A0 = torch.randn(n, n, dtype=torch.float64, device=dev) / math.sqrt(n)
-    A0 += 3.0 * torch.eye(n, dtype=torch.float64, device=dev)
-    A1 = torch.randn(n, n, dtype=torch.float64, device=dev) / math.sqrt(n)
+    AA, L, A0, A1 = _pair(n, 3, dev)
+    torch.manual_seed(3)
     g = torch.tensor([1.3, 1.3, 1.6, 1.6, 1.9, 1.9], dtype=torch.float64, device=dev)
     sizes = [60, 60, 60, 60]
     M = GroupedBlockJacobi(A0, A1, sizes, [1.4, 1.8], [(0, 4), (4, 6)], K)
-    op = GOperator(A0, A1, g)
+    op = GOperator(A0, A1, g, pair=(AA, L))
+    if use_lib:
+        op.cfg = op.LIB
     Bs = [torch.randn(n, K, dtype=torch.float64, device=dev) for _ in range(2)]
     side = torch.cuda.Stream()
     side.wait_stream(torch.cuda.current_stream())
