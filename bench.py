@@ -5,15 +5,19 @@ box triangles (n = 60,000 unknowns, the FP64 matrix is 28.8 GB) plus a
 676-triangle fault with uniform unit reverse slip, eps = 0.5 h, mu = 30 GPa,
 nu = 0.25 (synthetic geometry generated here; no data files).
 
-One step = the whole FP64 GPU solve from the mesh arrays resident in HBM:
-dense assembly (calibrated jump, fused row sums), the block-Jacobi
-preconditioner build (patch super-blocks forced to the block-Jacobi rung,
-9,000-DOF chunks, explicit chunk inverses), and FGMRES (restart 200) to the
-relative error target against an FP64 reference solution computed once,
-untimed, at rtol 1e-13.
+One step = a whole GPU solve from the mesh arrays: dense assembly
+(calibrated jump, fused row sums), the preconditioner build (block-Jacobi
+over cluster chunks, explicit chunk inverses) and the Krylov solve to the
+relative-error target against an FP64 reference solution computed once,
+untimed, at rtol 1e-13.  Two methods are stepped: plain FP64 (block-Jacobi
+FGMRES) and mixed (BF16 far field + FP64 near field inner FGMRES with FP32
+block-Jacobi and a recycled Krylov space, under an FP64 outer FGMRES);
+``value`` is the mixed step, the plain step and the ratio are in ``config``.
+The learned preconditioner's A/B on the same system is reported beside it.
 
-Prints one JSON line (see the keys at the end of ``main``).  ``--gpus`` above
-1 exits: this build measures one GPU.
+``--workload laplace`` (batched Laplace-state solves) and ``--workload
+readout`` (stress-readout throughput) print their own JSON lines.
+``--gpus`` above 1 exits: this build measures one GPU.
 """
 from __future__ import annotations
 
