@@ -144,10 +144,12 @@ def cpu_baseline(model, eps, system, n_iters, n_rows=48, gemv_rows=3000):
 
 def kernel_probe(n: int, reps: int):
     """Traffic probe for the PMC counters: ``reps`` launches of the FGMRES
-    matvec kernel on an n x n FP64 matrix, then ``reps`` plain device copies
-    of the same byte count (the counter calibration: a copy of B bytes reads
-    B and writes B).  Prints the per-launch algorithmic bytes."""
+    matvec kernel on an n x n FP64 matrix, ``reps`` launches of the mixed
+    method's BF16 far-field matvec on its BF16 copy, then ``reps`` plain
+    device copies of the FP64 matrix (the counter calibration: a copy of B
+    bytes reads B and writes B).  Prints the per-launch algorithmic bytes."""
     import torch
+    from mbem_amd._lib import call, ptr, stream
     from mbem_amd.la.solver import gemv
     dev = torch.device("cuda", 0)
     A = torch.randn((n, n), dtype=torch.float64, device=dev)
@@ -156,11 +158,19 @@ def kernel_probe(n: int, reps: int):
     for _ in range(reps):
         gemv(A, x, y)
     torch.cuda.synchronize()
+    ldf = (n + 7) // 8 * 8
+    F = torch.empty((n, ldf), dtype=torch.int16, device=dev)
+    call("mbem_cast_lo", ptr(A), n, n, n, ptr(F), ldf, 1, stream())
+    xf = x.float()
+    for _ in range(reps):
+        call("mbem_gemv_lo", ptr(F), ldf, n, n, ptr(xf), ptr(y), 1.0, 0.0, 1, stream())
+    torch.cuda.synchronize()
     B = torch.empty_like(A)
     for _ in range(reps):
         B.copy_(A)
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "gemv+copy", "n": n, "gemv_bytes": n * n * 8 + 2 * n * 8,
+    print(json.dumps({"probe": "gemv+gemv_lo+copy", "n": n, "gemv_bytes": n * n * 8 + 2 * n * 8,
+                      "gemv_lo_bytes": n * n * 2 + n * 4 + n * 8,
                       "copy_bytes_each_way": n * n * 8, "reps": reps}), flush=True)
 
 
