@@ -147,9 +147,10 @@ int mbem_invert_batched(double *M, int64_t m, int64_t batch, int32_t *ipiv, int3
 
 /* Dense LU of a row-major n x n operator in place and the solve A x = b:
  * the reference dense solve (backends/dense.py:532-571, la/lu.py:110-239 with
- * lu="double"). */
-int mbem_lu_factor(double *A, int64_t n, int32_t *ipiv, int32_t *info, hipStream_t stream);
-int mbem_lu_solve(const double *LU, int64_t n, const int32_t *ipiv, double *b, int64_t nrhs,
+ * lu="double").  64-bit sizes and pivots (ipiv n int64, info 1 int64 on the
+ * device): the dense path's matrices pass 2^31 elements. */
+int mbem_lu_factor(double *A, int64_t n, int64_t *ipiv, int64_t *info, hipStream_t stream);
+int mbem_lu_solve(const double *LU, int64_t n, const int64_t *ipiv, double *b, int64_t nrhs,
                   hipStream_t stream);
 
 /* Block-Jacobi apply (la/preconditioner.py:456-462 + _bj_solve :601-630):
@@ -227,8 +228,8 @@ int mbem_gemm_g(const double *A0, const double *A1, int64_t lda, int64_t m, int6
                 int64_t ldy, double alpha, double beta, int cfg, hipStream_t stream);
 
 /* Row-major C (m x n) = alpha A (m x k) B (k x n) + beta C through the
- * library DGEMM (rocBLAS, matrix cores): the autotuner's library candidate for
- * the batched operator, applied to [A0 | A1] [X ; X diag(g)]. */
+ * library DGEMM (rocBLAS, matrix cores, 64-bit sizes): the autotuner's library
+ * candidate for the batched operator, applied to [A0 | A1] [X ; X diag(g)]. */
 int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                         const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
                         hipStream_t stream);

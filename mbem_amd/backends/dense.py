@@ -483,8 +483,8 @@ class AssembledDense:
             else:
                 LU = self.A_dev.clone()
             m = LU.shape[0]
-            ipiv = torch.empty(m, dtype=torch.int32, device=dev)
-            info = torch.zeros(1, dtype=torch.int32, device=dev)
+            ipiv = torch.empty(m, dtype=torch.int64, device=dev)
+            info = torch.zeros(1, dtype=torch.int64, device=dev)
             t0 = time.perf_counter()
             call("mbem_lu_factor", ptr(LU), m, ptr(ipiv), ptr(info), stream())
             if int(info.item()) != 0:

@@ -511,15 +511,12 @@ int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const dou
     if (m < 0 || n < 0 || k < 0 || (m > 0 && n > 0 && (!A || !B || !C)) || lda < k || ldb < n || ldc < n)
         return MBEM_EINVAL;
     if (m == 0 || n == 0) return MBEM_OK;
-    if (m > INT32_MAX || n > INT32_MAX || k > INT32_MAX || lda > INT32_MAX || ldb > INT32_MAX || ldc > INT32_MAX)
-        return MBEM_EINVAL;
     if (!g_blas && rocblas_create_handle(&g_blas) != rocblas_status_success) return MBEM_EHIP;
     if (rocblas_set_stream(g_blas, stream) != rocblas_status_success) return MBEM_EHIP;
-    // row-major C (m x n) = A (m x k) B (k x n)  <=>  column-major C^T = B^T A^T
-    const rocblas_status st = rocblas_dgemm(g_blas, rocblas_operation_none, rocblas_operation_none,
-                                            (rocblas_int)n, (rocblas_int)m, (rocblas_int)k, &alpha, B,
-                                            (rocblas_int)ldb, A, (rocblas_int)lda, &beta, C,
-                                            (rocblas_int)ldc);
+    // row-major C (m x n) = A (m x k) B (k x n)  <=>  column-major C^T = B^T A^T;
+    // the 64-bit interface, since A (m x lda) passes 2^31 elements at n ~ 33,000
+    const rocblas_status st = rocblas_dgemm_64(g_blas, rocblas_operation_none, rocblas_operation_none,
+                                               n, m, k, &alpha, B, ldb, A, lda, &beta, C, ldc);
     return st == rocblas_status_success ? launch_status() : MBEM_EHIP;
 }
 

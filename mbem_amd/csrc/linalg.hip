@@ -256,7 +256,8 @@ int mbem_gather_block(const double* A, int64_t lda, const int64_t* idx, int64_t 
 // on the device, info: 1 int32 on the device.
 int mbem_invert(double* M, int64_t m, int32_t* ipiv, int32_t* info, hipStream_t stream)
 {
-    if (m < 0 || (m > 0 && (!M || !ipiv || !info)) || m > INT32_MAX) return MBEM_EINVAL;
+    // rocblas_int interface: the m x m array must stay below 2^31 elements
+    if (m < 0 || (m > 0 && (!M || !ipiv || !info)) || m * m > INT32_MAX) return MBEM_EINVAL;
     if (m == 0) return MBEM_OK;
     if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
     if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
@@ -272,7 +273,8 @@ int mbem_invert(double* M, int64_t m, int32_t* ipiv, int32_t* info, hipStream_t 
 int mbem_invert_batched(double* M, int64_t m, int64_t batch, int32_t* ipiv, int32_t* info,
                         hipStream_t stream)
 {
-    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info)) || m > INT32_MAX
+    // rocblas_int sizes per matrix (m * m < 2^31 elements); the batch stride is 64-bit
+    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info)) || m * m > INT32_MAX
         || batch > INT32_MAX)
         return MBEM_EINVAL;
     if (m == 0 || batch == 0) return MBEM_OK;
@@ -292,26 +294,27 @@ int mbem_invert_batched(double* M, int64_t m, int64_t batch, int32_t* ipiv, int3
 // LU factorization of a row-major n x n matrix in place (LAPACK getrf of its
 // column-major view, i.e. of A^T), and the matching solve A x = b (getrs with
 // transpose on that factor).  ipiv: n int32, info: 1 int32, both on device.
-int mbem_lu_factor(double* A, int64_t n, int32_t* ipiv, int32_t* info, hipStream_t stream)
+int mbem_lu_factor(double* A, int64_t n, int64_t* ipiv, int64_t* info, hipStream_t stream)
 {
-    if (n < 0 || (n > 0 && (!A || !ipiv || !info)) || n > INT32_MAX) return MBEM_EINVAL;
+    if (n < 0 || (n > 0 && (!A || !ipiv || !info))) return MBEM_EINVAL;
     if (n == 0) return MBEM_OK;
     if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
     if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
-    if (rocsolver_dgetrf(g_handle, (rocblas_int)n, (rocblas_int)n, A, (rocblas_int)n, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
+    // the 64-bit interface: at the dense path's sizes n^2 exceeds 2^31 elements,
+    // past what the rocblas_int interface indexes
+    if (rocsolver_dgetrf_64(g_handle, n, n, A, n, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
     return launch_status();
 }
 
-int mbem_lu_solve(const double* LU, int64_t n, const int32_t* ipiv, double* b, int64_t nrhs,
+int mbem_lu_solve(const double* LU, int64_t n, const int64_t* ipiv, double* b, int64_t nrhs,
                   hipStream_t stream)
 {
     if (n < 0 || nrhs < 0 || (n > 0 && (!LU || !ipiv || !b))) return MBEM_EINVAL;
     if (n == 0 || nrhs == 0) return MBEM_OK;
     if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
     if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
-    if (rocsolver_dgetrs(g_handle, rocblas_operation_transpose, (rocblas_int)n, (rocblas_int)nrhs,
-                         const_cast<double*>(LU), (rocblas_int)n, const_cast<int32_t*>(ipiv), b,
-                         (rocblas_int)n) != rocblas_status_success)
+    if (rocsolver_dgetrs_64(g_handle, rocblas_operation_transpose, n, nrhs, const_cast<double*>(LU), n,
+                            const_cast<int64_t*>(ipiv), b, n) != rocblas_status_success)
         return MBEM_EHIP;
     return launch_status();
 }

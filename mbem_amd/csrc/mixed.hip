@@ -414,7 +414,10 @@ int mbem_gather_block_f32(const double* A, int64_t lda, const int64_t* idx, int6
 int mbem_invert_batched_f32(float* M, int64_t m, int64_t batch, int32_t* ipiv, int32_t* info,
                             hipStream_t stream)
 {
-    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info)) || m > INT32_MAX) return MBEM_EINVAL;
+    // rocblas_int sizes per matrix (m * m < 2^31 elements); the batch stride is 64-bit
+    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info)) || m * m > INT32_MAX ||
+        batch > INT32_MAX)
+        return MBEM_EINVAL;
     if (m == 0 || batch == 0) return MBEM_OK;
     if (!g_handle32 && rocblas_create_handle(&g_handle32) != rocblas_status_success) return MBEM_EHIP;
     if (rocblas_set_stream(g_handle32, stream) != rocblas_status_success) return MBEM_EHIP;
