@@ -50,6 +50,13 @@ struct Work {
 // v_mfma_f64_16x16x4f64 operand layout (wave64), as measured on gfx950:
 // lane l holds A[i = l % 16][k = l / 16] and B[k = l / 16][j = l % 16], and
 // element v of its double4 result is D[i = l / 16 + 4 v][j = l % 16].
+//
+// Block tile BM x BN over WM x WN waves, depth slices of BK through a double
+// buffer in LDS: the next slice's global loads are issued before the MFMAs
+// of the current one and stored into the other buffer after them, one
+// barrier per slice.  (The double buffer and eight waves per 128 x 128 tile
+// measured 62 TFLOP/s at n = 31,104, K = 256, against 50 TFLOP/s single
+// buffered.)
 template <int BM, int BN, int BK, int WM, int WN, bool DUAL, bool TABLE>
 __global__ __launch_bounds__(WM * WN * 64) void k_mgemm(
     const double* __restrict__ A0, const double* __restrict__ A1, long lda, long M, long Kd,
@@ -63,14 +70,13 @@ __global__ __launch_bounds__(WM * WN * 64) void k_mgemm(
     constexpr int PADX = BN + 2;
     static_assert(TM >= 1 && TN >= 1, "tile shape");
     static_assert(BK % 4 == 0, "BK");
-    static_assert((BM * BK / 2) % NT == 0 && (BK * BN / 2) % NT == 0, "load split");
-    constexpr int LA = BM * BK / 2 / NT;      // dbl2 loads per thread per A tile
-    constexpr int LX = BK * BN / 2 / NT;      // dbl2 loads per thread per X tile
+    constexpr int NA2 = BM * BK / 2, NX2 = BK * BN / 2;      // dbl2 elements per tile
+    constexpr int LA = (NA2 + NT - 1) / NT, LX = (NX2 + NT - 1) / NT;
 
-    __shared__ double sA0[BK * PADA];
-    __shared__ double sA1[DUAL ? BK * PADA : 1];
-    __shared__ double sX[BK * PADX];
-    __shared__ double sXG[DUAL ? BK * PADX : 1];
+    __shared__ double sA0[2][BK * PADA];
+    __shared__ double sA1[2][DUAL ? BK * PADA : 1];
+    __shared__ double sX[2][BK * PADX];
+    __shared__ double sXG[2][DUAL ? BK * PADX : 1];
 
     long row0, col0;
     const double* Ab0 = A0;
@@ -115,7 +121,7 @@ __global__ __launch_bounds__(WM * WN * 64) void k_mgemm(
             const int r = e / (BK / 2), kk = (e % (BK / 2)) * 2;
             const long gr = row0 + r, gk = k0 + kk;
             dbl2 v0 = {0.0, 0.0}, v1 = {0.0, 0.0};
-            if (gr < Mr) {
+            if (e < NA2 && gr < Mr) {
                 if (gk + 1 < Kr) {
                     v0 = *reinterpret_cast<const dbl2*>(Ab0 + gr * ldar + gk);
                     if (DUAL) v1 = *reinterpret_cast<const dbl2*>(Ab1 + gr * ldar + gk);
@@ -133,60 +139,63 @@ __global__ __launch_bounds__(WM * WN * 64) void k_mgemm(
             const int k = e / (BN / 2), cc = (e % (BN / 2)) * 2;
             const long gk = k0 + k, gc = col0 + cc;
             dbl2 v = {0.0, 0.0};
-            if (gk < Kr) {
+            if (e < NX2 && gk < Kr) {
                 if (gc + 1 < Nr) v = *reinterpret_cast<const dbl2*>(Xb + gk * ldx + gc);
                 else if (gc < Nr) v.x = Xb[gk * ldx + gc];
             }
             rx[q] = v;
         }
     };
-    auto store_tiles = [&]() {
+    auto store_tiles = [&](int buf) {
 #pragma unroll
         for (int q = 0; q < LA; ++q) {
             const int e = tid + q * NT;
+            if (e >= NA2) continue;
             const int r = e / (BK / 2), kk = (e % (BK / 2)) * 2;
-            sA0[kk * PADA + r] = ra0[q].x;
-            sA0[(kk + 1) * PADA + r] = ra0[q].y;
+            sA0[buf][kk * PADA + r] = ra0[q].x;
+            sA0[buf][(kk + 1) * PADA + r] = ra0[q].y;
             if (DUAL) {
-                sA1[kk * PADA + r] = ra1[q].x;
-                sA1[(kk + 1) * PADA + r] = ra1[q].y;
+                sA1[buf][kk * PADA + r] = ra1[q].x;
+                sA1[buf][(kk + 1) * PADA + r] = ra1[q].y;
             }
         }
 #pragma unroll
         for (int q = 0; q < LX; ++q) {
             const int e = tid + q * NT;
+            if (e >= NX2) continue;
             const int k = e / (BN / 2), cc = (e % (BN / 2)) * 2;
-            sX[k * PADX + cc] = rx[q].x;
-            sX[k * PADX + cc + 1] = rx[q].y;
+            sX[buf][k * PADX + cc] = rx[q].x;
+            sX[buf][k * PADX + cc + 1] = rx[q].y;
             if (DUAL) {
                 const long gc = col0 + cc;
                 const double g0 = (gc < Nr) ? g[gc] : 0.0;
                 const double g1 = (gc + 1 < Nr) ? g[gc + 1] : 0.0;
-                sXG[k * PADX + cc] = rx[q].x * g0;
-                sXG[k * PADX + cc + 1] = rx[q].y * g1;
+                sXG[buf][k * PADX + cc] = rx[q].x * g0;
+                sXG[buf][k * PADX + cc + 1] = rx[q].y * g1;
             }
         }
     };
 
     const int fi = lane & 15, fk = lane >> 4;
     load_tiles(0);
+    store_tiles(0);
+    __syncthreads();
+    int cur = 0;
     for (long k0 = 0; k0 < Kr; k0 += BK) {
-        __syncthreads();
-        store_tiles();
-        __syncthreads();
-        if (k0 + BK < Kr) load_tiles(k0 + BK);      // prefetch the next depth slice
+        const bool more = k0 + BK < Kr;
+        if (more) load_tiles(k0 + BK);          // next slice in flight during the MFMAs
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 4) {
             double a0[TM], a1[DUAL ? TM : 1], b0[TN], b1[DUAL ? TN : 1];
 #pragma unroll
             for (int a = 0; a < TM; ++a) {
-                a0[a] = sA0[(kk + fk) * PADA + wrow + a * 16 + fi];
-                if (DUAL) a1[a] = sA1[(kk + fk) * PADA + wrow + a * 16 + fi];
+                a0[a] = sA0[cur][(kk + fk) * PADA + wrow + a * 16 + fi];
+                if (DUAL) a1[a] = sA1[cur][(kk + fk) * PADA + wrow + a * 16 + fi];
             }
 #pragma unroll
             for (int b = 0; b < TN; ++b) {
-                b0[b] = sX[(kk + fk) * PADX + wcol + b * 16 + fi];
-                if (DUAL) b1[b] = sXG[(kk + fk) * PADX + wcol + b * 16 + fi];
+                b0[b] = sX[cur][(kk + fk) * PADX + wcol + b * 16 + fi];
+                if (DUAL) b1[b] = sXG[cur][(kk + fk) * PADX + wcol + b * 16 + fi];
             }
 #pragma unroll
             for (int a = 0; a < TM; ++a)
@@ -197,6 +206,9 @@ __global__ __launch_bounds__(WM * WN * 64) void k_mgemm(
                         acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(a1[a], b1[b], acc[a][b], 0, 0, 0);
                 }
         }
+        if (more) store_tiles(cur ^ 1);         // the other buffer: last read two slices ago
+        __syncthreads();
+        cur ^= 1;
     }
     // write back: element v of tile (a, b) is row wrow + 16a + lane / 16 + 4 v,
     // column wcol + 16b + lane % 16
@@ -413,14 +425,14 @@ __global__ __launch_bounds__(256) void k_permute(const double* __restrict__ A, l
 
 struct Cfg { int bm, bn, bk, wm, wn; };
 constexpr Cfg kCfgs[] = {
-    {64, 64, 16, 2, 2},
-    {64, 128, 16, 2, 2},
-    {128, 64, 16, 2, 2},
-    {128, 128, 16, 2, 2},
-    {64, 256, 16, 1, 4},
-    {32, 256, 16, 1, 4},
-    {128, 128, 8, 2, 2},
     {64, 64, 8, 2, 2},
+    {64, 128, 8, 2, 2},
+    {128, 64, 8, 2, 2},
+    {128, 128, 8, 4, 2},
+    {128, 128, 8, 2, 4},
+    {64, 128, 8, 2, 4},
+    {128, 64, 8, 4, 2},
+    {128, 128, 16, 2, 2},
 };
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
@@ -441,14 +453,14 @@ int dispatch(int cfg, dim3 grid, hipStream_t st, const double* A0, const double*
 #define MBEM_CFG(i, BM, BN, BK, WM, WN) \
     case i: launch_cfg<BM, BN, BK, WM, WN, DUAL, TABLE>(grid, st, A0, A1, lda, M, Kd, X, ldx, N, g, Y, ldy, alpha, beta, work); break;
     switch (cfg) {
-        MBEM_CFG(0, 64, 64, 16, 2, 2)
-        MBEM_CFG(1, 64, 128, 16, 2, 2)
-        MBEM_CFG(2, 128, 64, 16, 2, 2)
-        MBEM_CFG(3, 128, 128, 16, 2, 2)
-        MBEM_CFG(4, 64, 256, 16, 1, 4)
-        MBEM_CFG(5, 32, 256, 16, 1, 4)
-        MBEM_CFG(6, 128, 128, 8, 2, 2)
-        MBEM_CFG(7, 64, 64, 8, 2, 2)
+        MBEM_CFG(0, 64, 64, 8, 2, 2)
+        MBEM_CFG(1, 64, 128, 8, 2, 2)
+        MBEM_CFG(2, 128, 64, 8, 2, 2)
+        MBEM_CFG(3, 128, 128, 8, 4, 2)
+        MBEM_CFG(4, 128, 128, 8, 2, 4)
+        MBEM_CFG(5, 64, 128, 8, 2, 4)
+        MBEM_CFG(6, 128, 64, 8, 4, 2)
+        MBEM_CFG(7, 128, 128, 16, 2, 2)
         default: return MBEM_EINVAL;
     }
 #undef MBEM_CFG
