@@ -540,13 +540,13 @@ def main():
     ap.add_argument("--fault-cells", type=int, nargs=2, default=(13, 26))
     ap.add_argument("--target-err", type=float, default=1e-8)
     ap.add_argument("--rtol", type=float, default=2e-11)
-    ap.add_argument("--bj-chunk", type=int, default=3000)
+    ap.add_argument("--bj-chunk", type=int, default=1500)
     ap.add_argument("--grouping", choices=("patch", "region"), default="region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=("plain", "mixed", "both"), default="both")
     ap.add_argument("--storage", choices=("bf16", "fp32"), default="bf16")
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
-    ap.add_argument("--bj-chunk-mixed", type=int, default=3000)
+    ap.add_argument("--bj-chunk-mixed", type=int, default=1500)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
     ap.add_argument("--recycle", type=int, default=60,
                     help="recycled vectors kept across the mixed method's inner solves")

@@ -145,6 +145,15 @@ int mbem_invert(double *M, int64_t m, int32_t *ipiv, int32_t *info, hipStream_t 
 int mbem_invert_batched(double *M, int64_t m, int64_t batch, int32_t *ipiv, int32_t *info,
                         hipStream_t stream);
 
+/* Padded chunk for one strided-batched inverse of unequal chunks: M (mp x mp,
+ * FP64 or FP32) = [A[idx, idx] 0; 0 I]; its inverse's leading m x m block is
+ * the chunk's inverse.  And a 2-D strided copy (4- or 8-byte elements) that
+ * moves that block into the packed inverse store. */
+int mbem_gather_block_pad(const double *A, int64_t lda, const int64_t *idx, int64_t m, int64_t mp,
+                          void *M, int f32, hipStream_t stream);
+int mbem_copy2d(const void *src, int64_t lds, void *dst, int64_t ldd, int64_t rows, int64_t cols,
+                int esz, hipStream_t stream);
+
 /* Dense LU of a row-major n x n operator in place and the solve A x = b:
  * the reference dense solve (backends/dense.py:532-571, la/lu.py:110-239 with
  * lu="double").  64-bit sizes and pivots (ipiv n int64, info 1 int64 on the
@@ -282,6 +291,10 @@ int mbem_gather_comb(const double *A0, const double *A1, int64_t lda, const int6
                      double g, double *out, hipStream_t stream);
 int mbem_permute(const double *A, int64_t lda, const int64_t *idx, int64_t n, double *out,
                  int64_t ldo, hipStream_t stream);
+/* The same chunk padded to mp x mp with an identity block, for one batched
+ * inverse over chunks of unequal size. */
+int mbem_gather_comb_pad(const double *A0, const double *A1, int64_t lda, const int64_t *idx,
+                         int64_t m, int64_t mp, double g, double *out, hipStream_t stream);
 
 /* HIP graphs: capture the launches made on a (non-null) stream, instantiate,
  * replay, destroy; and an async copy that a graph can hold. */

@@ -46,6 +46,8 @@ _SIGS = {
     "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert": [_p, _i64, _p, _p, _p],
     "mbem_invert_batched": [_p, _i64, _i64, _p, _p, _p],
+    "mbem_gather_block_pad": [_p, _i64, _p, _i64, _i64, _p, _int, _p],
+    "mbem_copy2d": [_p, _i64, _p, _i64, _i64, _i64, _int, _p],
     "mbem_lu_factor": [_p, _i64, _p, _p, _p],
     "mbem_lu_solve": [_p, _i64, _p, _p, _i64, _p],
     "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
@@ -77,6 +79,7 @@ _SIGS = {
     "mbem_fg_givens": [_i64, _i64, _i64, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p, _p],
     "mbem_fg_solve": [_i64, _i64, _p, _p, _p, _p, _p],
     "mbem_gather_comb": [_p, _p, _i64, _p, _i64, _d, _p, _p],
+    "mbem_gather_comb_pad": [_p, _p, _i64, _p, _i64, _i64, _d, _p, _p],
     "mbem_permute": [_p, _i64, _p, _i64, _p, _i64, _p],
     "mbem_graph_begin": [_p],
     "mbem_graph_end": [_p, _p],

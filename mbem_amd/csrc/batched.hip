@@ -410,6 +410,23 @@ __global__ __launch_bounds__(256) void k_gather_comb(const double* __restrict__ 
     out[t] = A0[src] + g * A1[src];
 }
 
+// padded chunk of B(g): out (mp x mp) = [A0 + g A1 on idx x idx, 0; 0, I]
+__global__ __launch_bounds__(256) void k_gather_comb_pad(const double* __restrict__ A0,
+                                                         const double* __restrict__ A1, long lda,
+                                                         const int64_t* __restrict__ idx, long m,
+                                                         long mp, double g, double* __restrict__ out)
+{
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mp * mp) return;
+    const long i = t / mp, j = t % mp;
+    if (i < m && j < m) {
+        const long src = idx[i] * lda + idx[j];
+        out[t] = A0[src] + g * A1[src];
+    } else {
+        out[t] = (i == j) ? 1.0 : 0.0;
+    }
+}
+
 // out[i, j] = A[idx[i], idx[j]] for a full permutation (m = n)
 __global__ __launch_bounds__(256) void k_permute(const double* __restrict__ A, long lda,
                                                  const int64_t* __restrict__ idx, long n,
@@ -629,6 +646,16 @@ int mbem_gather_comb(const double* A0, const double* A1, int64_t lda, const int6
     if (m == 0) return MBEM_OK;
     hipLaunchKernelGGL(k_gather_comb, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, stream, A0,
                        A1, lda, idx, m, g, out);
+    return launch_status();
+}
+
+int mbem_gather_comb_pad(const double* A0, const double* A1, int64_t lda, const int64_t* idx,
+                         int64_t m, int64_t mp, double g, double* out, hipStream_t stream)
+{
+    if (m < 0 || mp < m || (mp > 0 && (!out || (m > 0 && (!A0 || !A1 || !idx))))) return MBEM_EINVAL;
+    if (mp == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_gather_comb_pad, dim3((unsigned)((mp * mp + 255) / 256)), dim3(256), 0, stream,
+                       A0, A1, lda, idx, m, mp, g, out);
     return launch_status();
 }
 

@@ -143,6 +143,34 @@ __global__ __launch_bounds__(256) void k_gather_block(const double* __restrict__
     M[t] = A[idx[i] * lda + idx[j]];
 }
 
+// padded chunk for a batched inverse: M (mp x mp) = [A[idx, idx] 0; 0 I],
+// in FP64 or FP32 (F32 = true); the inverse's leading m x m block is the
+// chunk's inverse (no pivot ever leaves the leading block: the identity
+// rows are zero in its columns)
+template <bool F32>
+__global__ __launch_bounds__(256) void k_gather_pad(const double* __restrict__ A, long lda,
+                                                    const int64_t* __restrict__ idx, long m,
+                                                    long mp, void* __restrict__ Mv)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= mp * mp) return;
+    long i = t / mp, j = t % mp;
+    double v = (i < m && j < m) ? A[idx[i] * lda + idx[j]] : (i == j ? 1.0 : 0.0);
+    if (F32) reinterpret_cast<float*>(Mv)[t] = (float)v;
+    else reinterpret_cast<double*>(Mv)[t] = v;
+}
+
+// dst (rows x cols, ld ldd) = src (ld lds), elements of esz bytes (4 or 8)
+template <typename T>
+__global__ __launch_bounds__(256) void k_copy2d(const T* __restrict__ src, long lds, T* __restrict__ dst,
+                                                long ldd, long rows, long cols)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= rows * cols) return;
+    long i = t / cols, j = t % cols;
+    dst[i * ldd + j] = src[i * lds + j];
+}
+
 // block-Jacobi apply: wave w of the launch takes rows [row0, row0 + R) of
 // chunk wt[w] (table rows: chunk id, first row); z[idx[i]] = sum_j
 // inv[i, j] r[idx[j]] for each row i of that chunk.
@@ -248,6 +276,33 @@ int mbem_gather_block(const double* A, int64_t lda, const int64_t* idx, int64_t 
     if (m < 0 || (m > 0 && (!A || !idx || !M))) return MBEM_EINVAL;
     if (m == 0) return MBEM_OK;
     hipLaunchKernelGGL(k_gather_block, dim3((unsigned)((m * m + 255) / 256)), dim3(256), 0, stream, A, lda, idx, m, M);
+    return launch_status();
+}
+
+int mbem_gather_block_pad(const double* A, int64_t lda, const int64_t* idx, int64_t m, int64_t mp,
+                          void* M, int f32, hipStream_t stream)
+{
+    if (m < 0 || mp < m || (mp > 0 && (!M || (m > 0 && (!A || !idx))))) return MBEM_EINVAL;
+    if (mp == 0) return MBEM_OK;
+    const unsigned nb = (unsigned)((mp * mp + 255) / 256);
+    if (f32) hipLaunchKernelGGL(k_gather_pad<true>, dim3(nb), dim3(256), 0, stream, A, lda, idx, m, mp, M);
+    else hipLaunchKernelGGL(k_gather_pad<false>, dim3(nb), dim3(256), 0, stream, A, lda, idx, m, mp, M);
+    return launch_status();
+}
+
+int mbem_copy2d(const void* src, int64_t lds, void* dst, int64_t ldd, int64_t rows, int64_t cols,
+                int esz, hipStream_t stream)
+{
+    if (rows < 0 || cols < 0 || (esz != 4 && esz != 8) || (rows > 0 && cols > 0 && (!src || !dst)))
+        return MBEM_EINVAL;
+    if (rows == 0 || cols == 0) return MBEM_OK;
+    const unsigned nb = (unsigned)((rows * cols + 255) / 256);
+    if (esz == 8)
+        hipLaunchKernelGGL(k_copy2d<double>, dim3(nb), dim3(256), 0, stream, (const double*)src, lds,
+                           (double*)dst, ldd, rows, cols);
+    else
+        hipLaunchKernelGGL(k_copy2d<float>, dim3(nb), dim3(256), 0, stream, (const float*)src, lds,
+                           (float*)dst, ldd, rows, cols);
     return launch_status();
 }
 

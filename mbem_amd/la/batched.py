@@ -133,33 +133,38 @@ class GroupedBlockJacobi:
         self.inv = torch.zeros(len(self.anchors) * per_anchor + 2, dtype=F64, device=dev)
         self.a_off = np.zeros((len(self.anchors), len(self.sizes)), dtype=np.int64)
         idx_all = torch.arange(n, dtype=I64, device=dev)
-        off = 0
+        offs = np.concatenate([[0], np.cumsum([m * ld for m, ld in zip(self.sizes, self.ld)])])
+        sizes = np.asarray(self.sizes)
+        order = np.argsort(sizes, kind="stable")
         for a, g in enumerate(self.anchors):
-            # invert runs of equal-size chunks in one strided batch
-            k = 0
-            while k < len(self.sizes):
-                k1 = k
-                while k1 + 1 < len(self.sizes) and self.sizes[k1 + 1] == self.sizes[k]:
-                    k1 += 1
-                m, cnt = self.sizes[k], k1 - k + 1
-                tmp = torch.empty((cnt, m, m), dtype=F64, device=dev)
-                for q in range(cnt):
-                    o = int(self.offsets[k + q])
-                    call("mbem_gather_comb", ptr(A0), ptr(A1), A0.stride(0), ptr(idx_all[o:o + m]),
-                         m, g, ptr(tmp[q]), stream())
-                ipiv = torch.empty((cnt, m), dtype=torch.int32, device=dev)
+            base = a * int(offs[-1])
+            self.a_off[a] = base + offs[:-1]
+            # size classes (largest / smallest <= 1.1), each padded with an
+            # identity block to its largest size and inverted in one batch
+            c0 = 0
+            while c0 < len(order):
+                c1 = c0
+                while c1 + 1 < len(order) and sizes[order[c1 + 1]] <= 1.1 * sizes[order[c0]]:
+                    c1 += 1
+                cls = order[c0:c1 + 1]
+                mp, cnt = int(sizes[cls].max()), len(cls)
+                tmp = torch.empty((cnt, mp, mp), dtype=F64, device=dev)
+                for q, k in enumerate(cls):
+                    o, m = int(self.offsets[k]), int(sizes[k])
+                    call("mbem_gather_comb_pad", ptr(A0), ptr(A1), A0.stride(0),
+                         ptr(idx_all[o:o + m]), m, mp, g, ptr(tmp[q]), stream())
+                ipiv = torch.empty((cnt, mp), dtype=torch.int32, device=dev)
                 info = torch.empty(cnt, dtype=torch.int32, device=dev)
-                call("mbem_invert_batched", ptr(tmp), m, cnt, ptr(ipiv), ptr(info), stream())
+                call("mbem_invert_batched", ptr(tmp), mp, cnt, ptr(ipiv), ptr(info), stream())
                 if int(info.abs().max().item()) != 0:
                     raise np.linalg.LinAlgError(f"block-Jacobi chunk of B(g={g:.6g}) is singular")
-                for q in range(cnt):
-                    ld = self.ld[k + q]
-                    self.a_off[a, k + q] = off
-                    dst = self.inv[off:off + m * ld].view(m, ld)
-                    dst[:, :m].copy_(tmp[q])
-                    off += m * ld
+                for q, k in enumerate(cls):
+                    m = int(sizes[k])
+                    call("mbem_copy2d", ptr(tmp[q]), mp, ptr(self.inv[int(self.a_off[a, k]):]),
+                         self.ld[k], m, m, 8, stream())
                 del tmp
-                k = k1 + 1
+                c0 = c1 + 1
+        off = len(self.anchors) * int(offs[-1])
         torch.cuda.current_stream().synchronize()
         self.build_s = time.perf_counter() - t0
         self.inverse_bytes = 8 * off

@@ -201,28 +201,41 @@ class BlockGaussSeidel:
         R = _lib.BJ_ROWS_PER_WAVE
         tables = {k: [] for k in range(len(self.sbs))}
         for ci, (k, dofs) in enumerate(chunks):
-            m = dofs.size
-            M = self.inv[int(iptr[ci]):int(iptr[ci + 1])]
-            call("mbem_gather_block_f32" if f32 else "mbem_gather_block", ptr(A), A.stride(0),
-                 ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), m, ptr(M), stream())
-            tables[k] += [(ci, r0) for r0 in range(0, m, R)]
-        # invert runs of equal-size chunks (contiguous in ``inv``) as one batch
-        ci = 0
-        while ci < len(chunks):
-            cj = ci
-            while cj + 1 < len(chunks) and sizes[cj + 1] == sizes[ci]:
-                cj += 1
-            m, nb = int(sizes[ci]), cj - ci + 1
-            ipiv = torch.empty(nb * m, dtype=torch.int32, device=dev)
+            tables[k] += [(ci, r0) for r0 in range(0, int(dofs.size), R)]
+        # invert the chunks in size classes (largest / smallest <= 1.1): each
+        # class is padded to its largest size with an identity block and
+        # inverted in one strided-batched getrf + getri, then the leading
+        # blocks are packed into ``inv``
+        order = np.argsort(sizes, kind="stable")
+        esz = 4 if f32 else 8
+        c0 = 0
+        while c0 < len(order):
+            c1 = c0
+            while c1 + 1 < len(order) and sizes[order[c1 + 1]] <= 1.1 * sizes[order[c0]]:
+                c1 += 1
+            cls = order[c0:c1 + 1]
+            mp, nb = int(sizes[cls].max()), len(cls)
+            tmp = torch.empty((nb, mp, mp), dtype=torch.float32 if f32 else F64, device=dev)
+            for q, ci in enumerate(cls):
+                call("mbem_gather_block_pad", ptr(A), A.stride(0),
+                     ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), int(sizes[ci]), mp,
+                     ptr(tmp[q]), int(f32), stream())
+            ipiv = torch.empty(nb * mp, dtype=torch.int32, device=dev)
             info = torch.zeros(nb, dtype=torch.int32, device=dev)
-            call("mbem_invert_batched_f32" if f32 else "mbem_invert_batched",
-                 ptr(self.inv[int(iptr[ci]):]), m, nb, ptr(ipiv), ptr(info), stream())
+            call("mbem_invert_batched_f32" if f32 else "mbem_invert_batched", ptr(tmp), mp, nb,
+                 ptr(ipiv), ptr(info), stream())
             bad = info.cpu().numpy()
             if np.any(bad != 0):
-                k = int(np.flatnonzero(bad)[0])
-                raise np.linalg.LinAlgError(f"block-Jacobi chunk {ci + k} ({m} DOFs) is singular "
-                                            f"(getrf info {int(bad[k])})")
-            ci = cj + 1
+                q = int(np.flatnonzero(bad)[0])
+                raise np.linalg.LinAlgError(f"block-Jacobi chunk {int(cls[q])} "
+                                            f"({int(sizes[cls[q]])} DOFs) is singular "
+                                            f"(getrf info {int(bad[q])})")
+            for q, ci in enumerate(cls):
+                m = int(sizes[ci])
+                call("mbem_copy2d", ptr(tmp[q]), mp, ptr(self.inv[int(iptr[ci]):]), m, m, m, esz,
+                     stream())
+            del tmp
+            c0 = c1 + 1
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
             sb.wave_table = torch.as_tensor(wt, dtype=I64, device=dev).contiguous()
