@@ -544,7 +544,7 @@ def main():
     ap.add_argument("--grouping", choices=("patch", "region"), default="region")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--method", choices=("plain", "mixed", "both"), default="both")
-    ap.add_argument("--storage", choices=("bf16", "fp32"), default="bf16")
+    ap.add_argument("--storage", choices=("bf16", "fp32", "int8"), default="int8")
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=1500)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
@@ -698,8 +698,8 @@ def main():
             "workload": f"box 40x40x20 km, N={info['box_triangles']} P0 triangles "
                         f"(n={n}), dipping fault {info['fault_triangles']} triangles, "
                         f"eps=0.5h, mu=30 GPa nu=0.25",
-            "method": ("mixed: BF16 far field + FP64 near field on an FP64 outer FGMRES, "
-                       "FP32 block-Jacobi" if main_label == "mixed" else
+            "method": (f"mixed: {args.storage.upper()} far field + FP64 near field on an FP64 "
+                       f"outer FGMRES, FP32 block-Jacobi" if main_label == "mixed" else
                        "plain FP64: GPU assembly + block-Jacobi FGMRES"),
             "storage": args.storage if main_label == "mixed" else "fp64",
             "rtol": args.rtol, "rel_err_vs_fp64_reference": max(errs),

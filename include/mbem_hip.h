@@ -39,6 +39,9 @@ extern "C" {
 /* rows of a block-Jacobi chunk one wavefront applies (mbem_bj_apply tables) */
 #define MBEM_BJ_ROWS_PER_WAVE 4
 
+/* rows of the int8 far field one wavefront multiplies (mbem_gemv_i8) */
+#define MBEM_I8_ROWS_PER_WAVE 4
+
 /* doubles per source record */
 int mbem_src_fields(void);
 
@@ -129,6 +132,11 @@ int mbem_multiaxpy(const double *V, int64_t ldv, int64_t k, int64_t n, const dou
 /* y = a x + b y */
 int mbem_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t stream);
 
+/* w /= ||w|| with ||w||^2 = *sumsq read on the device, skipped when the norm
+ * is not above 1e-300: the Arnoldi normalization V[j+1] = w / H[j+1, j]
+ * (la/solver.py:183-184). */
+int mbem_normalize(int64_t n, const double *sumsq, double *w, hipStream_t stream);
+
 /* M (m,m) = A[idx, idx]: a block-Jacobi chunk of the assembled operator
  * (la/preconditioner.py:445 eval_block(ch, ch)). */
 int mbem_gather_block(const double *A, int64_t lda, const int64_t *idx, int64_t m, double *M,
@@ -201,6 +209,29 @@ int mbem_vec_to_f32(const double *x, int64_t n, float *xf, hipStream_t stream);
  * accumulation, FP64 y. */
 int mbem_gemv_lo(const void *F, int64_t ldf, int64_t m, int64_t n, const float *x, double *y,
                  double alpha, double beta, int bf16, hipStream_t stream);
+
+/* int8 far-field storage: F[r, c] = scale[r * nseg + c / 64] * (out[r, c] - 128)
+ * with offset-binary bytes out = q + 128, q = rint(A / s) clamped to
+ * [-127, 127], and one FP32 scale s = (float)(max |A| / 127) per row and
+ * 64-column segment (q = 0 where s = 0), nseg = mbem_i8_segments(n); out
+ * 16-byte aligned with ldo a multiple of 16.  Same role as mbem_cast_lo
+ * (la/solver.py:178 operator, stored low). */
+int64_t mbem_i8_segments(int64_t n);
+int mbem_cast_i8(const double *A, int64_t lda, int64_t m, int64_t n, uint8_t *out, int64_t ldo,
+                 float *scale, int64_t nseg, hipStream_t stream);
+
+/* Hold the near blocks out of A while its int8 copy is built: restore == 0
+ * copies them to vals (9 doubles per block, as mbem_near_extract) and zeroes
+ * them in A; restore != 0 writes vals back, leaving A as it was. */
+int mbem_near_hold(double *A, int64_t lda, int64_t nbr, const int64_t *rowptr, const int64_t *col,
+                   int64_t nnzb, double *vals, int restore, hipStream_t stream);
+
+/* y = alpha F x + beta y for the int8 far field above (ldf a multiple of
+ * 16, F and x 16-byte aligned), FP32 x, FP32 products and accumulation,
+ * FP64 y. */
+int mbem_gemv_i8(const uint8_t *F, int64_t ldf, const float *scale, int64_t nseg, int64_t m,
+                 int64_t n, const float *x, double *y, double alpha, double beta,
+                 hipStream_t stream);
 
 /* y += alpha N x for a 3x3 block-CSR N in FP64 (the near field). */
 int mbem_bsr_gemv(int64_t nbr, const int64_t *rowptr, const int64_t *col, const double *vals,

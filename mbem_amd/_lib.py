@@ -43,6 +43,7 @@ _SIGS = {
     "mbem_multidot": [_p, _i64, _i64, _i64, _p, _p, _p, _p],
     "mbem_multiaxpy": [_p, _i64, _i64, _i64, _p, _p, _d, _p],
     "mbem_axpby": [_i64, _d, _p, _d, _p, _p],
+    "mbem_normalize": [_i64, _p, _p, _p],
     "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert": [_p, _i64, _p, _p, _p],
     "mbem_invert_batched": [_p, _i64, _i64, _p, _p, _p],
@@ -56,6 +57,10 @@ _SIGS = {
     "mbem_near_extract": [_p, _i64, _i64, _p, _p, _i64, _p, _p, _i64, _int, _p],
     "mbem_vec_to_f32": [_p, _i64, _p, _p],
     "mbem_gemv_lo": [_p, _i64, _i64, _i64, _p, _p, _d, _d, _int, _p],
+    "mbem_i8_segments": [_i64],
+    "mbem_cast_i8": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p],
+    "mbem_near_hold": [_p, _i64, _i64, _p, _p, _i64, _p, _int, _p],
+    "mbem_gemv_i8": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
     "mbem_bsr_gemv": [_i64, _p, _p, _p, _p, _p, _d, _p],
     "mbem_gather_block_f32": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert_batched_f32": [_p, _i64, _i64, _p, _p, _p],
@@ -105,7 +110,7 @@ _SIGS = {
     "mbem_lr_outer": [_p, _i64, _p, _i64, _i64, _i64, _i64, _p, _p],
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }
-_RESTYPE = {"mbem_multidot_work": _i64}
+_RESTYPE = {"mbem_multidot_work": _i64, "mbem_i8_segments": _i64}
 
 _lib = None
 

@@ -132,6 +132,17 @@ __global__ __launch_bounds__(256) void k_axpby(long n, double a, const double* _
     y[j] = (b == 0.0) ? a * x[j] : a * x[j] + b * y[j];
 }
 
+// w /= sqrt(*sumsq) when that norm exceeds 1e-300; the squared norm is read
+// on the device, so the host need not see it before the next launch
+__global__ __launch_bounds__(256) void k_normalize(long n, const double* __restrict__ sumsq,
+                                                   double* __restrict__ w)
+{
+    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    const double nrm = sqrt(fmax(*sumsq, 0.0));
+    if (nrm > 1e-300) w[j] = w[j] / nrm;
+}
+
 // chunk gather: M[i, j] = A[idx[i], idx[j]]
 __global__ __launch_bounds__(256) void k_gather_block(const double* __restrict__ A, long lda,
                                                       const int64_t* __restrict__ idx, long m,
@@ -267,6 +278,14 @@ int mbem_axpby(int64_t n, double a, const double* x, double b, double* y, hipStr
     if (n < 0 || (n > 0 && (!x || !y))) return MBEM_EINVAL;
     if (n == 0) return MBEM_OK;
     hipLaunchKernelGGL(k_axpby, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, a, x, b, y);
+    return launch_status();
+}
+
+int mbem_normalize(int64_t n, const double* sumsq, double* w, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!sumsq || !w))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_normalize, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, n, sumsq, w);
     return launch_status();
 }
 

@@ -3,9 +3,9 @@
 // The FP64 matrix A splits as A = F + N: N holds the near-field 3x3 blocks
 // (source triangle within k (eps + h) of the collocation point, self blocks
 // included) in FP64 block-CSR; F is every other entry, stored in BF16 or FP32
-// in a dense row-major copy whose near entries are zero.  y = F x + N x:
-// the far part reads 2 (or 4) bytes per entry and accumulates in FP32, the
-// near part is exact FP64.  Also here: the FP32 block-Jacobi inverses
+// (or int8 with per-segment scales) in a dense row-major copy whose near
+// entries are zero.  y = F x + N x: the far part reads 2, 4 (or 1) bytes per
+// entry and accumulates in FP32, the near part is exact FP64.  Also here: the FP32 block-Jacobi inverses
 // (strided-batched sgetrf/sgetri) and their apply.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
@@ -187,6 +187,195 @@ __global__ __launch_bounds__(256) void k_gemv_lo(const void* __restrict__ F, lon
             const float a = BF ? bf16_lo((unsigned int)Fb[rows[r] + c]) : Ff[rows[r] + c];
             acc[r] += a * xc;
         }
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        float t = wave_sum_f(acc[r]);
+        if (lane == 0 && r0 + r < m) {
+            double* p = y + r0 + r;
+            *p = (beta == 0.0) ? alpha * (double)t : alpha * (double)t + beta * (*p);
+        }
+    }
+}
+
+// ---- int8 far field ------------------------------------------------------
+// F[r, c] = s[r, c / 64] * (u[r, c] - 128): offset-binary bytes u = q + 128,
+// q = rint(A / s) in [-127, 127], and one FP32 scale s = max|A| / 127 per row
+// and 64-column segment.  Every entry's error is at most s / 2, a fixed
+// fraction of the segment's largest entry (a floating 8-bit format instead
+// errs by a fixed fraction of each entry, largest on the largest entries).
+constexpr long kSeg = 64;
+
+// one wave per (row, 1024 columns), in 4 passes of 256: lane l encodes 4
+// consecutive columns, so a 64-column segment is one 16-lane row of the wave
+// and its largest magnitude is a reduction inside that row; loads (32 bytes
+// a lane) and 4-byte stores are contiguous across the wave
+__global__ __launch_bounds__(256) void k_cast_i8(const double* __restrict__ A, long lda, long m,
+                                                 long n, unsigned char* __restrict__ out, long ldo,
+                                                 float* __restrict__ scale, long nseg)
+{
+    const int lane = threadIdx.x & 63;
+    const long nblk = (n + 1023) / 1024;
+    const long w = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (w >= m * nblk) return;
+    const long r = w / nblk;
+    const long cb = (w % nblk) * 1024;
+    const double* a = A + r * lda;
+    unsigned char* o = out + r * ldo;
+    // 16-byte loads when the row start is 16-byte aligned (A itself is)
+    const bool vec = (lda & 1) == 0 && (reinterpret_cast<uintptr_t>(A) & 15) == 0;
+#pragma unroll
+    for (int p = 0; p < 4; ++p) {
+        const long c = cb + 256 * p + 4 * lane;
+        double v[4];
+        if (vec && c + 3 < n) {
+            const dbl2 a01 = *reinterpret_cast<const dbl2*>(a + c);
+            const dbl2 a23 = *reinterpret_cast<const dbl2*>(a + c + 2);
+            v[0] = a01.x; v[1] = a01.y; v[2] = a23.x; v[3] = a23.y;
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) v[i] = (c + i < n) ? a[c + i] : 0.0;
+        }
+        double amax = fmax(fmax(fabs(v[0]), fabs(v[1])), fmax(fabs(v[2]), fabs(v[3])));
+        amax = fmax(amax, __shfl_xor(amax, 1, 64));
+        amax = fmax(amax, __shfl_xor(amax, 2, 64));
+        amax = fmax(amax, __shfl_xor(amax, 4, 64));
+        amax = fmax(amax, __shfl_xor(amax, 8, 64));
+        const float s = (float)(amax / 127.0);
+        unsigned int q[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            double t = (s > 0.0f) ? rint(v[i] / (double)s) : 0.0;
+            t = fmin(fmax(t, -127.0), 127.0);
+            q[i] = (unsigned int)((int)t + 128);
+        }
+        if (c + 3 < n) {
+            *reinterpret_cast<unsigned int*>(o + c) = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+        } else {
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                if (c + i < n) o[c + i] = (unsigned char)q[i];
+        }
+        if ((lane & 15) == 0 && c < n) scale[r * nseg + c / kSeg] = s;
+    }
+}
+
+// near blocks of A held out while the int8 copy is built: RESTORE = false
+// copies them to vals and zeroes them in A; RESTORE = true writes them back
+template <bool RESTORE>
+__global__ __launch_bounds__(256) void k_near_hold(double* __restrict__ A, long lda, long nbr,
+                                                   const int64_t* __restrict__ rowptr,
+                                                   const int64_t* __restrict__ col,
+                                                   double* __restrict__ vals)
+{
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    long nnzb = rowptr[nbr];
+    if (t >= 9 * nnzb) return;
+    long b = t / 9;
+    int q = (int)(t % 9), a = q / 3, c = q % 3;
+    long lo_i = 0, hi_i = nbr;
+    while (hi_i - lo_i > 1) {
+        long mid = (lo_i + hi_i) >> 1;
+        if (rowptr[mid] <= b) lo_i = mid; else hi_i = mid;
+    }
+    double* p = A + (3 * lo_i + a) * lda + 3 * col[b] + c;
+    if (RESTORE) {
+        *p = vals[t];
+    } else {
+        vals[t] = *p;
+        *p = 0.0;
+    }
+}
+
+__device__ __forceinline__ float ub0(unsigned int w) { return (float)(w & 0xffu); }
+__device__ __forceinline__ float ub1(unsigned int w) { return (float)((w >> 8) & 0xffu); }
+__device__ __forceinline__ float ub2(unsigned int w) { return (float)((w >> 16) & 0xffu); }
+__device__ __forceinline__ float ub3(unsigned int w) { return (float)(w >> 24); }
+
+// sum over 16 offset-binary bytes (one 16-byte load) of u_j x_j
+__device__ __forceinline__ float u8_dot16(const u32x4 w, const flt4 a, const flt4 b, const flt4 c,
+                                          const flt4 d)
+{
+    return ub0(w.x) * a.x + ub1(w.x) * a.y + ub2(w.x) * a.z + ub3(w.x) * a.w + ub0(w.y) * b.x +
+           ub1(w.y) * b.y + ub2(w.y) * b.z + ub3(w.y) * b.w + ub0(w.z) * c.x + ub1(w.z) * c.y +
+           ub2(w.z) * c.z + ub3(w.z) * c.w + ub0(w.w) * d.x + ub1(w.w) * d.y + ub2(w.w) * d.z +
+           ub3(w.w) * d.w;
+}
+
+__device__ __forceinline__ float sum16(const flt4 a, const flt4 b, const flt4 c, const flt4 d)
+{
+    return ((a.x + a.y) + (a.z + a.w)) + ((b.x + b.y) + (b.z + b.w)) + ((c.x + c.y) + (c.z + c.w)) +
+           ((d.x + d.y) + (d.z + d.w));
+}
+
+// y = alpha (F x) + beta y, F int8 with per-segment scales, x FP32: one wave
+// per R rows with U 16-byte loads per row in flight (columns 1024 apart).  The
+// row bases are wave-uniform (scalar registers) and each load is that base
+// plus one 32-bit per-lane offset, which keeps the address arithmetic off the
+// vector ALU.  A lane's 16 columns lie in one 64-column segment, so its
+// partial sum is s (sum u x - 128 sum x) with the lane's own scale.
+template <int R, int U>
+__global__ __launch_bounds__(256) void k_gemv_i8(const unsigned char* __restrict__ F, long ldf,
+                                                 const float* __restrict__ S, long nseg, long m,
+                                                 long n, const float* __restrict__ x,
+                                                 double* __restrict__ y, double alpha, double beta)
+{
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long r0 = ((long)blockIdx.x * (blockDim.x >> 6) + wv) * R;
+    if (r0 >= m) return;
+    const unsigned char* fb[R];
+    const float* sb[R];
+    float acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        acc[r] = 0.0f;
+        const long rr = (r0 + r < m) ? r0 + r : m - 1;
+        fb[r] = F + rr * ldf;
+        sb[r] = S + rr * nseg;
+    }
+    // columns within a row fit 32 bits (the entry point checks n < 2^31)
+    const unsigned nvec = (unsigned)(n / 16);
+    unsigned v = lane;
+    for (; v + 64 * (U - 1) < nvec; v += 64 * U) {
+        u32x4 w[U][R];
+        float s[U][R];
+        flt4 xv[U][4];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned c = (v + 64 * u) * 16;
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                w[u][r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(fb[r] + c));
+                s[u][r] = sb[r][c / kSeg];
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) xv[u][q] = reinterpret_cast<const flt4*>(x + c)[q];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const float o = 128.0f * sum16(xv[u][0], xv[u][1], xv[u][2], xv[u][3]);
+#pragma unroll
+            for (int r = 0; r < R; ++r)
+                acc[r] += s[u][r] * (u8_dot16(w[u][r], xv[u][0], xv[u][1], xv[u][2], xv[u][3]) - o);
+        }
+    }
+    for (; v < nvec; v += 64) {
+        const unsigned c = v * 16;
+        const flt4* xp = reinterpret_cast<const flt4*>(x + c);
+        const flt4 xa = xp[0], xb = xp[1], xc = xp[2], xd = xp[3];
+        const float o = 128.0f * sum16(xa, xb, xc, xd);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[r] += sb[r][c / kSeg] *
+                      (u8_dot16(__builtin_nontemporal_load(reinterpret_cast<const u32x4*>(fb[r] + c)),
+                                xa, xb, xc, xd) - o);
+    }
+    for (unsigned c = nvec * 16 + lane; c < (unsigned)n; c += 64) {
+        const float xc = x[c];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[r] += sb[r][c / kSeg] * (((float)fb[r][c] - 128.0f) * xc);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -390,6 +579,54 @@ int mbem_gemv_lo(const void* F, int64_t ldf, int64_t m, int64_t n, const float* 
         hipLaunchKernelGGL((k_gemv_lo<R, true>), grid, block, 0, stream, F, ldf, m, n, x, y, alpha, beta);
     else
         hipLaunchKernelGGL((k_gemv_lo<R, false>), grid, block, 0, stream, F, ldf, m, n, x, y, alpha, beta);
+    return launch_status();
+}
+
+int64_t mbem_i8_segments(int64_t n) { return n <= 0 ? 0 : (n + kSeg - 1) / kSeg; }
+
+int mbem_near_hold(double* A, int64_t lda, int64_t nbr, const int64_t* rowptr, const int64_t* col,
+                   int64_t nnzb, double* vals, int restore, hipStream_t stream)
+{
+    if (nbr < 0 || nnzb < 0 || (nnzb > 0 && (!A || !rowptr || !col || !vals))) return MBEM_EINVAL;
+    if (nnzb == 0) return MBEM_OK;
+    dim3 grid((unsigned)((9 * nnzb + 255) / 256)), block(256);
+    if (restore)
+        hipLaunchKernelGGL((k_near_hold<true>), grid, block, 0, stream, A, lda, nbr, rowptr, col, vals);
+    else
+        hipLaunchKernelGGL((k_near_hold<false>), grid, block, 0, stream, A, lda, nbr, rowptr, col, vals);
+    return launch_status();
+}
+
+int mbem_cast_i8(const double* A, int64_t lda, int64_t m, int64_t n, uint8_t* out, int64_t ldo,
+                 float* scale, int64_t nseg, hipStream_t stream)
+{
+    if (m < 0 || n < 0 || (m > 0 && n > 0 && (!A || !out || !scale || lda < n || ldo < n)))
+        return MBEM_EINVAL;
+    if (nseg != mbem_i8_segments(n)) return MBEM_EINVAL;
+    if (m == 0 || n == 0) return MBEM_OK;
+    // 16-byte stores of the packed codes
+    if ((reinterpret_cast<uintptr_t>(out) & 15) || (ldo % 16)) return MBEM_EINVAL;
+    const long waves = m * ((n + 1023) / 1024);
+    const long blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffL) return MBEM_EINVAL;
+    hipLaunchKernelGGL(k_cast_i8, dim3((unsigned)blocks), dim3(256), 0, stream, A, lda, m, n, out, ldo, scale, nseg);
+    return launch_status();
+}
+
+int mbem_gemv_i8(const uint8_t* F, int64_t ldf, const float* scale, int64_t nseg, int64_t m,
+                 int64_t n, const float* x, double* y, double alpha, double beta, hipStream_t stream)
+{
+    if (m < 0 || n < 0 || (m > 0 && (!F || !y || !scale || ldf < n)) || (n > 0 && !x)) return MBEM_EINVAL;
+    // the scale rows hold at least the segments the n columns touch
+    if (nseg < mbem_i8_segments(n)) return MBEM_EINVAL;
+    if (m == 0) return MBEM_OK;
+    if ((reinterpret_cast<uintptr_t>(F) & 15) || (reinterpret_cast<uintptr_t>(x) & 15) || (ldf % 16))
+        return MBEM_EINVAL;
+    if (n > INT32_MAX) return MBEM_EINVAL;     // 32-bit column offsets
+    constexpr int R = MBEM_I8_ROWS_PER_WAVE;
+    long waves = (m + R - 1) / R;
+    dim3 grid((unsigned)((waves + 3) / 4)), block(256);
+    hipLaunchKernelGGL((k_gemv_i8<R, 4>), grid, block, 0, stream, F, ldf, scale, nseg, m, n, x, y, alpha, beta);
     return launch_status();
 }
 

@@ -5,8 +5,9 @@ Operator split A = F + N on the device:
      (collocation point, source triangle) pair closer than
      ``k_near * (eps + h)`` of the source, where the closed-form log and
      arctangent terms cancel;
-  F  everything else in BF16 (or FP32) storage, near entries zeroed,
-     multiplied with FP32 accumulation.
+  F  everything else in BF16, FP32 or int8 storage, near entries zeroed,
+     multiplied with FP32 accumulation (int8: offset-binary bytes with one
+     FP32 scale per row and 64-column segment).
 
 Solve: the outer loop is FP64 FGMRES on the exact FP64 operator (Arnoldi,
 Givens and the true residual all FP64), so it is iterative refinement; its
@@ -73,22 +74,17 @@ class MixedOperator:
     """y = F x + N x on the device (see the module docstring)."""
 
     def __init__(self, assembled, storage: str = "bf16", k_near: float = 2.0, pattern=None):
-        if storage not in ("bf16", "fp32"):
-            raise ValueError(f"storage {storage!r} (expected 'bf16' or 'fp32')")
+        if storage not in ("bf16", "fp32", "int8"):
+            raise ValueError(f"storage {storage!r} (expected 'bf16', 'fp32' or 'int8')")
         _lib.require_gpu()
         A = assembled.A_dev
         n = A.shape[0]
         dev = A.device
         self.n = n
         self.bf16 = storage == "bf16"
+        self.i8 = storage == "int8"
         self.storage = storage
         t0 = time.perf_counter()
-        align = 8 if self.bf16 else 4
-        self.ldf = ((n + align - 1) // align) * align
-        dt = torch.int16 if self.bf16 else torch.float32
-        self.F = torch.empty((n, self.ldf), dtype=dt, device=dev)
-        call("mbem_cast_lo", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf, int(self.bf16),
-             stream())
         rowptr, col = pattern if pattern is not None else near_structure(assembled, k_near)
         self.nbr = rowptr.size - 1
         self.nnzb = int(rowptr[-1])
@@ -96,19 +92,47 @@ class MixedOperator:
         self.rowptr = torch.as_tensor(rowptr, dtype=I64, device=dev)
         self.col = torch.as_tensor(col, dtype=I64, device=dev)
         self.vals = torch.empty(9 * max(self.nnzb, 1), dtype=F64, device=dev)
-        call("mbem_near_extract", ptr(A), A.stride(0), self.nbr, ptr(self.rowptr), ptr(self.col),
-             self.nnzb, ptr(self.vals), ptr(self.F), self.ldf, int(self.bf16), stream())
+        align = {"bf16": 8, "fp32": 4, "int8": 16}[storage]
+        self.ldf = ((n + align - 1) // align) * align
+        if self.i8:
+            # near blocks held out of A (saved to vals, zeroed) so the segment
+            # scales see the far field only, then written back
+            self.nseg = int(_lib.load().mbem_i8_segments(n))
+            self.F = torch.empty((n, self.ldf), dtype=torch.uint8, device=dev)
+            self.S = torch.empty((n, max(self.nseg, 1)), dtype=torch.float32, device=dev)
+            hold = (self.nbr, ptr(self.rowptr), ptr(self.col), self.nnzb, ptr(self.vals))
+            call("mbem_near_hold", ptr(A), A.stride(0), *hold, 0, stream())
+            call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf, ptr(self.S),
+                 self.nseg, stream())
+            call("mbem_near_hold", ptr(A), A.stride(0), *hold, 1, stream())
+            self.far_bytes = n * n + n * self.nseg * 4
+        else:
+            dt = torch.int16 if self.bf16 else torch.float32
+            self.F = torch.empty((n, self.ldf), dtype=dt, device=dev)
+            call("mbem_cast_lo", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf, int(self.bf16),
+                 stream())
+            call("mbem_near_extract", ptr(A), A.stride(0), self.nbr, ptr(self.rowptr),
+                 ptr(self.col), self.nnzb, ptr(self.vals), ptr(self.F), self.ldf, int(self.bf16),
+                 stream())
+            self.far_bytes = n * n * (2 if self.bf16 else 4)
         self._lower = {}
         self.xf = torch.empty(max(n, 4), dtype=torch.float32, device=dev)   # FP32 copy of x
         torch.cuda.current_stream().synchronize()
         self.build_s = time.perf_counter() - t0
-        self.far_bytes = n * n * (2 if self.bf16 else 4)
         self.near_bytes = self.nnzb * (72 + 8)
+
+    def _far(self, r0, r1, ncols, alpha, beta, out):
+        """out[:] = alpha F[r0:r1, :ncols] xf[:ncols] + beta out (xf already set)."""
+        if self.i8:
+            call("mbem_gemv_i8", ptr(self.F[r0:r1]), self.ldf, ptr(self.S[r0:r1]), self.nseg,
+                 r1 - r0, ncols, ptr(self.xf), ptr(out), alpha, beta, stream())
+        else:
+            call("mbem_gemv_lo", ptr(self.F[r0:r1]), self.ldf, r1 - r0, ncols, ptr(self.xf),
+                 ptr(out), alpha, beta, int(self.bf16), stream())
 
     def apply_into(self, v, out):
         call("mbem_vec_to_f32", ptr(v), self.n, ptr(self.xf), stream())
-        call("mbem_gemv_lo", ptr(self.F), self.ldf, self.n, self.n, ptr(self.xf), ptr(out), 1.0,
-             0.0, int(self.bf16), stream())
+        self._far(0, self.n, self.n, 1.0, 0.0, out)
         call("mbem_bsr_gemv", self.nbr, ptr(self.rowptr), ptr(self.col), ptr(self.vals), ptr(v),
              ptr(out), 1.0, stream())
         return out
@@ -138,10 +162,8 @@ class MixedOperator:
 
     def lower_into(self, r0, r1, z, out):
         """out -= (F + N)[r0:r1, :r0] z[:r0]: the Gauss-Seidel lower coupling."""
-        base = self.F[r0:r1]
         call("mbem_vec_to_f32", ptr(z), r0, ptr(self.xf), stream())
-        call("mbem_gemv_lo", ptr(base), self.ldf, r1 - r0, r0, ptr(self.xf), ptr(out), -1.0, 1.0,
-             int(self.bf16), stream())
+        self._far(r0, r1, r0, -1.0, 1.0, out)
         nbr, rowptr, col, vals = self._lower_bsr(r0, r1)
         call("mbem_bsr_gemv", nbr, ptr(rowptr), ptr(col), ptr(vals), ptr(z), ptr(out), -1.0,
              stream())
@@ -182,7 +204,9 @@ class _RecycledInner:
         self.C = torch.zeros((self.kmax, n), dtype=F64, device=dev)
         self.k = 0
         self.ws = _Krylov(n, maxiter, dev)
-        self.hc = torch.zeros(max(self.kmax, 1), dtype=F64, device=dev)
+        # row j: the C-projection coefficients of Arnoldi step j (kept on the
+        # device, read once per solve)
+        self.hc = torch.zeros((maxiter, max(self.kmax, 1)), dtype=F64, device=dev)
         self.cwork = torch.empty(int(_lib.load().mbem_multidot_work(max(self.kmax, 1), n)),
                                  dtype=F64, device=dev)
         self.iterations = []
@@ -208,7 +232,6 @@ class _RecycledInner:
         V, Z = ws.V, ws.Z
         H = np.zeros((m + 1, m))
         Hraw = np.zeros((m + 1, m))
-        Hc = np.zeros((max(k, 1), m))
         cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
         j_done = 0
         if beta / r_norm > self.rtol:
@@ -219,17 +242,14 @@ class _RecycledInner:
                 w = V[j + 1]
                 self.op.apply_into(Z[j], w)
                 if k > 0:
-                    call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(w),
-                         ptr(self.hc), ptr(self.cwork), stream())
-                    ws.axpy_rows(self.C, k, self.hc, w, -1.0)
-                hsum, wnorm = ws.cgs2(V, j + 1, w)
-                if k > 0:
-                    Hc[:k, j] = self.hc[:k].cpu().numpy()
+                    hc = self.hc[j]
+                    call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(w), ptr(hc),
+                         ptr(self.cwork), stream())
+                    ws.axpy_rows(self.C, k, hc, w, -1.0)
+                hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True)
                 H[:j + 1, j] = hsum
                 H[j + 1, j] = wnorm
                 Hraw[:j + 2, j] = H[:j + 2, j]
-                if wnorm > 1e-300:
-                    axpby(1.0 / wnorm, w, 0.0, w)
                 for i in range(j):
                     t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
                     H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
@@ -249,12 +269,14 @@ class _RecycledInner:
         if j_done == 0:
             return z
         y = _solve_upper(H[:j_done, :j_done], g[:j_done])
-        yd = torch.as_tensor(y, dtype=F64, device=x.device)
-        ws.axpy_rows(Z, j_done, yd, x, 1.0)
+        Hc = self.hc[:j_done, :k].cpu().numpy().T if k > 0 else None     # k x j_done
+        # x += Z y - U (Hc y): one upload of both coefficient vectors
+        coefs = np.concatenate([y, Hc @ y]) if k > 0 else y
+        cd = torch.as_tensor(coefs, dtype=F64).to(x.device, non_blocking=False)
+        ws.axpy_rows(Z, j_done, cd[:j_done], x, 1.0)
         if k > 0:
-            cy = torch.as_tensor(Hc[:k, :j_done] @ y, dtype=F64, device=x.device)
-            ws.axpy_rows(self.U, k, cy, x, -1.0)
-        self._grow(j_done, Hraw[:j_done + 1, :j_done], Hc[:k, :j_done] if k > 0 else None)
+            ws.axpy_rows(self.U, k, cd[j_done:], x, -1.0)
+        self._grow(j_done, Hraw[:j_done + 1, :j_done], Hc)
         return z
 
     def _grow(self, j, Hbar, Hc):
@@ -268,19 +290,23 @@ class _RecycledInner:
         Rinv = np.linalg.inv(R)
         ws, k, dev = self.ws, self.k, self.U.device
         V, Z = ws.V, ws.Z
+        # one upload of every coefficient column: rows of Q^T, Rinv^T, Hc^T
+        parts = [Q[:, :add].T, Rinv[:, :add].T] + ([Hc.T] if Hc is not None and k > 0 else [])
+        cf = torch.as_tensor(np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in parts])),
+                             dtype=F64).to(dev)
+        q_t = cf[:add * (j + 1)].view(add, j + 1)
+        r_t = cf[add * (j + 1):add * (2 * j + 1)].view(add, j)
         W = torch.empty((j, self.n), dtype=F64, device=dev)
         W.copy_(Z[:j])
         if Hc is not None and k > 0:
+            h_t = cf[add * (2 * j + 1):].view(j, k)
             for i in range(j):
-                ws.axpy_rows(self.U, k, torch.as_tensor(Hc[:, i], dtype=F64, device=dev), W[i],
-                             -1.0)
+                ws.axpy_rows(self.U, k, h_t[i], W[i], -1.0)
         for i in range(add):
             self.C[k + i].zero_()
-            ws.axpy_rows(V, j + 1, torch.as_tensor(Q[:, i], dtype=F64, device=dev), self.C[k + i],
-                         1.0)
+            ws.axpy_rows(V, j + 1, q_t[i], self.C[k + i], 1.0)
             self.U[k + i].zero_()
-            ws.axpy_rows(W, j, torch.as_tensor(Rinv[:, i], dtype=F64, device=dev), self.U[k + i],
-                         1.0)
+            ws.axpy_rows(W, j, r_t[i], self.U[k + i], 1.0)
         self.k = k + add
 
 

@@ -251,6 +251,12 @@ class BlockGaussSeidel:
         A, rr = self.A, self._rr
         for sb in self.sbs:
             seg = slice(sb.start, sb.stop)
+            if sb.start == 0:
+                # the first super-block has no lower coupling: read r itself
+                call("mbem_bj_apply_f32" if self.inverse_dtype == "f32" else "mbem_bj_apply",
+                     ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr), ptr(self.idx),
+                     ptr(self.inv_ptr), ptr(self.inv), ptr(r), ptr(z), stream())
+                continue
             rr[seg].copy_(r[seg])
             if sb.start > 0:
                 if self.coupling is not None:

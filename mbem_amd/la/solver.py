@@ -116,14 +116,18 @@ class _Krylov:
              ptr(self.work), stream())
         return out
 
-    def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor):
+    def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor, normalize: bool = False):
         """Project w against V[:k] twice (classical Gram-Schmidt, reorthogonalized)
-        and return (h, ||w||) on the host after ONE synchronization."""
+        and return (h, ||w||) on the host after ONE synchronization; with
+        ``normalize`` w is also divided by its norm on the device (when the
+        norm exceeds 1e-300) before that synchronization."""
         h1 = self.dots(V, k, w, 0)
         self.axpy_rows(V, k, h1, w, -1.0)
         h2 = self.dots(V, k, w, k)
         self.axpy_rows(V, k, h2, w, -1.0)
         self.dots(w.view(1, -1), 1, w, 2 * k)
+        if normalize:
+            call("mbem_normalize", self.n, ptr(self.h[2 * k:]), ptr(w), stream())
         hv = self.h[:2 * k + 1].cpu().numpy()
         return hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0)))
 
@@ -179,10 +183,15 @@ def fgmres(A, b, M=None, x0=None,
     n_pc = 0
     r = torch.empty(n, dtype=F64, device=dev)
 
+    x_is_zero = x0 is None
     while total_iters < maxiter and not stagnated:
-        A_mv(x, r)
-        n_mv += 1
-        axpby(1.0, b, -1.0, r)                       # r = b - A x
+        if x_is_zero:
+            r.copy_(b)                               # r = b - A 0, no matvec
+            x_is_zero = False
+        else:
+            A_mv(x, r)
+            n_mv += 1
+            axpby(1.0, b, -1.0, r)                   # r = b - A x
         beta = ws.norm(r)
         arnoldi_res = beta / b_norm
         if arnoldi_res < rtol:
@@ -205,12 +214,11 @@ def fgmres(A, b, M=None, x0=None,
             w = V[j + 1]
             A_mv(Z[j], w)
             n_mv += 1
-            # classical Gram-Schmidt, twice; one host synchronization
-            hsum, wnorm = ws.cgs2(V, j + 1, w)
+            # classical Gram-Schmidt, twice, and V[j+1] = w / ||w|| (when
+            # ||w|| > 1e-300) on the device; one host synchronization
+            hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True)
             H[:j + 1, j] = hsum
             H[j + 1, j] = wnorm
-            if H[j + 1, j] > 1e-300:
-                axpby(1.0 / H[j + 1, j], w, 0.0, w)
             # Givens (solver.py:187-205)
             for i in range(j):
                 t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]

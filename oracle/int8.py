@@ -1,0 +1,46 @@
+"""CPU oracle of the int8 far-field storage (test infrastructure only:
+imported by tests, never by the product path).
+
+The storage has no counterpart in the reference (whose FGMRES operator is the
+FP64 matrix, la/solver.py:178); this restates what mbem_amd/csrc/mixed.hip
+does, so the GPU bytes, scales and products can be checked bit for bit:
+
+  for row r and 64-column segment k, s = float32(max |A[r, seg]| / 127);
+  q = rint(A / s) (ties to even, the division in float64) clamped to
+  [-127, 127], q = 0 where s = 0; the byte is q + 128; the represented value
+  is s (byte - 128).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+SEG = 64
+
+
+def segments(n: int) -> int:
+    return -(-n // SEG) if n > 0 else 0
+
+
+def cast(A) -> tuple:
+    """(codes (m, n) uint8, scales (m, nseg) float32) of a float64 matrix."""
+    A = np.asarray(A, dtype=np.float64)
+    m, n = A.shape
+    nseg = segments(n)
+    codes = np.empty((m, n), dtype=np.uint8)
+    scales = np.empty((m, nseg), dtype=np.float32)
+    for k in range(nseg):
+        blk = A[:, k * SEG:(k + 1) * SEG]
+        s = (np.abs(blk).max(axis=1) / 127.0).astype(np.float32)
+        scales[:, k] = s
+        sd = s.astype(np.float64)[:, None]
+        with np.errstate(divide="ignore", invalid="ignore"):
+            q = np.where(sd > 0, np.rint(blk / np.where(sd > 0, sd, 1.0)), 0.0)
+        codes[:, k * SEG:(k + 1) * SEG] = (np.clip(q, -127, 127) + 128).astype(np.uint8)
+    return codes, scales
+
+
+def dequantize(codes, scales) -> np.ndarray:
+    """float64 matrix the int8 storage represents."""
+    n = codes.shape[1]
+    seg = np.arange(n) // SEG
+    return (codes.astype(np.float64) - 128.0) * scales[:, seg].astype(np.float64)

@@ -4,7 +4,13 @@ the 2,048-triangle box with its dipping fault.
   * the near-field blocks are copied exactly (FP64) and the low copy is zero
     there, so (F + N) equals A exactly on the near blocks;
   * the mixed matvec's error against the FP64 matvec is of the storage's
-    rounding: below 2e-3 relative (BF16 far field) and 2e-7 (FP32);
+    rounding: below 2e-3 relative (BF16 far field), 2e-7 (FP32) and 2e-3
+    (int8 with 64-column segment scales);
+  * the int8 codes and scales equal the oracle's (oracle/int8.py) bit for
+    bit, the int8 product equals the oracle's dequantized matrix times the
+    FP32 vector to FP32 accumulation (1e-5 of the sum of |terms|), and
+    holding the near blocks out of A and writing them back leaves A
+    bit-identical;
   * the nested FGMRES (FP64 outer on the exact operator) reaches the FP64
     true residual 1e-12 and the oracle's LU solution to 1e-9, with and
     without the recycled space across inner solves; the recycled pairs keep
@@ -31,7 +37,7 @@ def c1():
     return dict(asm=asm, A=A, b=b, x_lu=od.lu_solve(A, b))
 
 
-@pytest.mark.parametrize("storage,tol", [("bf16", 2e-3), ("fp32", 2e-7)])
+@pytest.mark.parametrize("storage,tol", [("bf16", 2e-3), ("fp32", 2e-7), ("int8", 2e-3)])
 def test_mixed_matvec(c1, storage, tol):
     import torch
     from mbem_amd.la.mixed import MixedOperator
@@ -52,16 +58,88 @@ def test_mixed_matvec(c1, storage, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("recycle", [0, 60])
-def test_mixed_solve(c1, recycle):
+@pytest.mark.parametrize("storage,recycle", [("bf16", 0), ("bf16", 60), ("int8", 60)])
+def test_mixed_solve(c1, storage, recycle):
     from mbem_amd.la.mixed import solve_mixed
     asm = c1["asm"]
-    x, rep = solve_mixed(asm, rtol=1e-12, storage="bf16", recycle=recycle)
+    x, rep = solve_mixed(asm, rtol=1e-12, storage=storage, recycle=recycle)
     xh = x.cpu().numpy()
     err = np.linalg.norm(xh - c1["x_lu"]) / np.linalg.norm(c1["x_lu"])
     print(f"{rep}; rel err vs oracle LU {err:.2e}")
     assert rep.converged and rep.outer.true_relres < 1e-12
     assert err < 1e-9
+
+
+@pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030)])
+def test_int8_cast_and_product_match_oracle(m, n):
+    """Random rows spanning 25 decades, a row 1e-11 smaller, an all-zero
+    segment, and column counts that leave a partial last segment, a partial
+    last 1024-column block and a tail past the last 16 bytes."""
+    import torch
+    from mbem_amd._lib import call, load, ptr, stream
+    from oracle import int8 as oi
+    rng = np.random.default_rng(m + n)
+    A = rng.standard_normal((m, n)) * np.exp(rng.uniform(-20, 5, n))[None, :]
+    A[m // 2] *= 1e-11
+    A[0, :64] = 0.0
+    dev = torch.device("cuda")
+    lda = n + 3
+    Ad = torch.zeros((m, lda), dtype=torch.float64, device=dev)
+    Ad[:, :n] = torch.as_tensor(A, device=dev)
+    nseg = int(load().mbem_i8_segments(n))
+    assert nseg == oi.segments(n)
+    ldf = ((n + 15) // 16) * 16
+    F = torch.zeros((m, ldf), dtype=torch.uint8, device=dev)
+    S = torch.zeros((m, nseg), dtype=torch.float32, device=dev)
+    call("mbem_cast_i8", ptr(Ad), lda, m, n, ptr(F), ldf, ptr(S), nseg, stream())
+    codes, scales = oi.cast(A)
+    assert np.array_equal(F[:, :n].cpu().numpy(), codes)
+    assert np.array_equal(S.cpu().numpy(), scales)
+    x = rng.standard_normal(n).astype(np.float32)
+    xd = torch.as_tensor(x, device=dev)
+    y0 = rng.standard_normal(m)
+    y = torch.as_tensor(y0, device=dev)
+    call("mbem_gemv_i8", ptr(F), ldf, ptr(S), nseg, m, n, ptr(xd), ptr(y), -0.5, 2.0, stream())
+    D = oi.dequantize(codes, scales)
+    ref = -0.5 * (D @ x.astype(np.float64)) + 2.0 * y0
+    # the kernel sums s (u x) - s (128 x) per 16 columns: the bar is FP32
+    # rounding of those terms
+    terms = (np.abs(codes.astype(np.float64)) + 128.0) * scales[:, np.arange(n) // 64]
+    scale = 0.5 * (terms @ np.abs(x.astype(np.float64))) + 2.0 * np.abs(y0)
+    err = np.abs(y.cpu().numpy() - ref) / scale
+    print(f"int8 {m}x{n}: product err / |terms| max {err.max():.2e}")
+    assert err.max() < 1e-5
+
+
+def test_near_hold_round_trip(c1):
+    import torch
+    from mbem_amd._lib import call, ptr, stream
+    from mbem_amd.la.mixed import near_structure
+    asm = c1["asm"]
+    A = asm.A_dev
+    before = A.clone()
+    rowptr, col = near_structure(asm, 2.0)
+    dev = A.device
+    rp = torch.as_tensor(rowptr, device=dev)
+    cl = torch.as_tensor(col, device=dev)
+    nnzb = int(rowptr[-1])
+    vals = torch.empty(9 * nnzb, dtype=torch.float64, device=dev)
+    call("mbem_near_hold", ptr(A), A.stride(0), rowptr.size - 1, ptr(rp), ptr(cl), nnzb, ptr(vals),
+         0, stream())
+    held = A.clone()
+    call("mbem_near_hold", ptr(A), A.stride(0), rowptr.size - 1, ptr(rp), ptr(cl), nnzb, ptr(vals),
+         1, stream())
+    assert torch.equal(A, before)
+    Bh = before.cpu().numpy()
+    Hh = held.cpu().numpy()
+    mask = np.zeros(Bh.shape, dtype=bool)
+    v = vals.view(-1, 3, 3).cpu().numpy()
+    for i in range(rowptr.size - 1):
+        for b in range(rowptr[i], rowptr[i + 1]):
+            j = col[b]
+            mask[3 * i:3 * i + 3, 3 * j:3 * j + 3] = True
+            assert np.array_equal(v[b], Bh[3 * i:3 * i + 3, 3 * j:3 * j + 3])
+    assert np.all(Hh[mask] == 0.0) and np.array_equal(Hh[~mask], Bh[~mask])
 
 
 def test_recycled_pairs_stay_consistent(c1):
