@@ -37,14 +37,15 @@ HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
 
-def _measured_traffic_ratio():
-    """HBM bytes / algorithmic bytes of the matvec kernel from the committed
+def _measured_traffic_ratio(kernel: str = "k_gemv"):
+    """HBM bytes / algorithmic bytes of a matvec kernel from the committed
     rocprofv3 PMC summary (FETCH_SIZE / WRITE_SIZE scaled on a plain copy),
     or None when no such summary is present."""
     import glob
     import re
     for path in sorted(glob.glob(os.path.join(ROOT, "profiles", "matvec_traffic_n*.txt"))):
-        m = re.search(r"traffic_over_algorithmic = ([0-9.eE+-]+)", open(path).read())
+        m = re.search(r"^" + re.escape(kernel) + r" traffic_over_algorithmic = ([0-9.eE+-]+)",
+                      open(path).read(), re.M)
         if m:
             return float(m.group(1)), os.path.basename(path)
     return None, None
@@ -58,6 +59,12 @@ def _executed_flops():
         return {}
     return {m.group(1): float(m.group(2)) for m in
             re.finditer(r"executed_flops_per_pair (\w+) = ([0-9.]+)", open(path).read())}
+
+
+def i8_bytes(n: int) -> int:
+    """Algorithmic bytes of one int8 far-field matvec (n x n): the codes, one
+    FP32 scale per row and 64 columns, x in FP32, y in FP64."""
+    return n * n + 4 * n * (-(-n // 64)) + 4 * n + 8 * n
 
 
 def _events_time(fn, reps):
@@ -145,11 +152,12 @@ def cpu_baseline(model, eps, system, n_iters, n_rows=48, gemv_rows=3000):
 def kernel_probe(n: int, reps: int):
     """Traffic probe for the PMC counters: ``reps`` launches of the FGMRES
     matvec kernel on an n x n FP64 matrix, ``reps`` launches of the mixed
-    method's BF16 far-field matvec on its BF16 copy, then ``reps`` plain
-    device copies of the FP64 matrix (the counter calibration: a copy of B
-    bytes reads B and writes B).  Prints the per-launch algorithmic bytes."""
+    method's BF16 far-field matvec on its BF16 copy and of the int8 one on
+    its int8 copy, then ``reps`` plain device copies of the FP64 matrix (the
+    counter calibration: a copy of B bytes reads B and writes B).  Prints the
+    per-launch algorithmic bytes."""
     import torch
-    from mbem_amd._lib import call, ptr, stream
+    from mbem_amd._lib import call, load, ptr, stream
     from mbem_amd.la.solver import gemv
     dev = torch.device("cuda", 0)
     A = torch.randn((n, n), dtype=torch.float64, device=dev)
@@ -165,12 +173,25 @@ def kernel_probe(n: int, reps: int):
     for _ in range(reps):
         call("mbem_gemv_lo", ptr(F), ldf, n, n, ptr(xf), ptr(y), 1.0, 0.0, 1, stream())
     torch.cuda.synchronize()
+    del F
+    ld8 = (n + 15) // 16 * 16
+    nseg = int(load().mbem_i8_segments(n))
+    F8 = torch.empty((n, ld8), dtype=torch.uint8, device=dev)
+    S8 = torch.empty((n, nseg), dtype=torch.float32, device=dev)
+    call("mbem_cast_i8", ptr(A), n, n, n, ptr(F8), ld8, ptr(S8), nseg, stream())
+    for _ in range(reps):
+        call("mbem_gemv_i8", ptr(F8), ld8, ptr(S8), nseg, n, n, ptr(xf), ptr(y), 1.0, 0.0,
+             stream())
+    torch.cuda.synchronize()
+    del F8, S8
     B = torch.empty_like(A)
     for _ in range(reps):
         B.copy_(A)
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "gemv+gemv_lo+copy", "n": n, "gemv_bytes": n * n * 8 + 2 * n * 8,
+    print(json.dumps({"probe": "gemv+gemv_lo+gemv_i8+copy", "n": n,
+                      "gemv_bytes": n * n * 8 + 2 * n * 8,
                       "gemv_lo_bytes": n * n * 2 + n * 4 + n * 8,
+                      "gemv_i8_bytes": i8_bytes(n),
                       "copy_bytes_each_way": n * n * 8, "reps": reps}), flush=True)
 
 
@@ -621,6 +642,21 @@ def main():
     y = torch.empty(n, dtype=torch.float64, device=A.device)
     gemv_ms = _events_time(lambda: gemv(A, x_ref, y), 10)
     gemv_bytes = n * n * 8 + 2 * n * 8
+    far = None
+    if args.method in ("mixed", "both"):
+        # the mixed method's far-field matvec, the mixed step's dominant kernel
+        from mbem_amd._lib import call, ptr, stream
+        from mbem_amd.la.mixed import MixedOperator
+        op = MixedOperator(asm, storage=args.storage)
+        call("mbem_vec_to_f32", ptr(x_ref), n, ptr(op.xf), stream())
+        far_ms = _events_time(lambda: op._far(0, n, n, 1.0, 0.0, y), 10)
+        far_bytes = {"int8": i8_bytes(n), "bf16": 2 * n * n + 12 * n,
+                     "fp32": 4 * n * n + 12 * n}[args.storage]
+        kname = {"int8": "k_gemv_i8", "bf16": "k_gemv_lo", "fp32": "k_gemv_lo"}[args.storage]
+        far = (kname, far_ms, far_bytes)
+        print(f"[bench] {kname} ({args.storage}) {far_ms:.3f} ms = "
+              f"{far_bytes / far_ms / 1e6:.0f} GB/s", flush=True)
+        del op
     asm = None
     torch.cuda.empty_cache()
     print(f"[bench] gemv {gemv_ms:.3f} ms = {gemv_bytes / gemv_ms / 1e6:.0f} GB/s setup "
@@ -680,7 +716,20 @@ def main():
         it_plain = results["plain"][2] if "plain" in results else [60]
         cpu = cpu_baseline(model, eps, system, int(np.median(it_plain)))
         print(f"[bench] cpu baseline {cpu['value']:.1f} s on {cpu['cores']} cores", flush=True)
-    ratio, src_name = _measured_traffic_ratio()
+    def roofline(kernel, ms, nbytes, what):
+        ratio, src_name = _measured_traffic_ratio(kernel)
+        return {"bound": "hbm", "achieved": nbytes / ms / 1e6, "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": nbytes / ms / 1e6 / HBM_PEAK_GBS,
+                "traffic": (None if ratio is None else ratio * nbytes),
+                "traffic_source": src_name, "kernel": f"{kernel} ({what})",
+                "algorithmic_bytes": nbytes, "avg_ms": ms}
+
+    roof_fp64 = roofline("k_gemv", gemv_ms, gemv_bytes, "FGMRES matvec, n x n FP64")
+    if main_label == "mixed" and far is not None:
+        roof_main = roofline(far[0], far[1], far[2],
+                             f"mixed far-field matvec, n x n {args.storage}")
+    else:
+        roof_main = roof_fp64
     line = {
         "metric": METRIC,
         "value": ms_per_step / 1e3,
@@ -715,13 +764,8 @@ def main():
                               f"inner solves"),
             "learned_preconditioner": learned,
         },
-        "roofline": {"bound": "hbm", "achieved": gemv_bytes / gemv_ms / 1e6,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": gemv_bytes / gemv_ms / 1e6 / HBM_PEAK_GBS,
-                     "traffic": (None if ratio is None else ratio * gemv_bytes),
-                     "traffic_source": src_name,
-                     "kernel": "k_gemv (FGMRES matvec, n x n FP64)",
-                     "algorithmic_bytes": gemv_bytes, "avg_ms": gemv_ms},
+        "roofline": roof_main,
+        "roofline_fp64_matvec": roof_fp64,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
