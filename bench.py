@@ -324,55 +324,14 @@ def bench_laplace(args):
     print(json.dumps(line), flush=True)
 
 
-def readout_workload(n_points, seed_pts=11):
-    """Sources: the N = 20,000 box + 676-triangle fault mesh, per-element
-    eps = 0.5 h, random unit densities (seed 3).  Points: half uniform in the
-    box interior away from the faces (0.1 h), half within 3 eps of the fault
-    plane (uniform over the fault, normal offset U(-3 eps, 3 eps))."""
-    from mbem_amd.cases import box_fault_model
-    from mbem_amd.kernels import basis as kb
-    model, _, info = box_fault_model(50, fault_cells=(13, 26))
-    region = model.regions[0]
-    patches = list(region.patches) + list(region.faults)
-    tv, nrm, eps = [], [], []
-    for p in patches:
-        a, b = kb._source_arrays(p.mesh)
-        tv.append(a)
-        nrm.append(b)
-        eps.append(0.5 * kb.element_sizes(p.mesh))
-    tv, nrm, eps = np.concatenate(tv), np.concatenate(nrm), np.concatenate(eps)
-    dens = np.random.default_rng(3).normal(size=(tv.shape[0], 3))
-    dens /= np.linalg.norm(dens, axis=1, keepdims=True)
-    rng = np.random.default_rng(seed_pts)
-    h = 40.0 / 50
-    nb = n_points // 2
-    lo = np.array([-20.0, -20.0, -20.0]) + 0.1 * h
-    hi = np.array([20.0, 20.0, 0.0]) - 0.1 * h
-    box_pts = lo + (hi - lo) * rng.random((nb, 3))
-    fault = region.faults[0]
-    fv = fault.mesh.vertices[fault.mesh.triangles]
-    area = 0.5 * np.linalg.norm(np.cross(fv[:, 1] - fv[:, 0], fv[:, 2] - fv[:, 0]), axis=1)
-    nf = n_points - nb
-    tri = rng.choice(fv.shape[0], size=nf, p=area / area.sum())
-    u, v = rng.random(nf), rng.random(nf)
-    flip = u + v > 1
-    u[flip], v[flip] = 1 - u[flip], 1 - v[flip]
-    base = fv[tri, 0] + u[:, None] * (fv[tri, 1] - fv[tri, 0]) + v[:, None] * (fv[tri, 2] - fv[tri, 0])
-    fn_ = np.cross(fv[0, 1] - fv[0, 0], fv[0, 2] - fv[0, 0])
-    fn_ /= np.linalg.norm(fn_)
-    eps_f = 0.5 * float(kb.element_sizes(fault.mesh).mean())
-    fault_pts = base + rng.uniform(-3 * eps_f, 3 * eps_f, size=(nf, 1)) * fn_[None, :]
-    pts = np.concatenate([box_pts, fault_pts])
-    return pts, tv, nrm, eps, dens, info
-
-
 def bench_readout(args):
     """Stress-readout throughput: dd_stress_contract - eigenstress_contract,
     FP64, full sum, 10^6 points x 20,676 sources.  Prints one JSON line."""
     import torch
     from mbem_amd.kernels import device as dv
     mu, lam = 30.0, 30.0
-    pts, tv, nrm, eps, dens, info = readout_workload(args.readout_points)
+    from mbem_amd.cases import readout_case
+    pts, tv, nrm, eps, dens, info = readout_case(args.readout_points)
     ns = tv.shape[0]
     print(f"[readout] points={pts.shape[0]} sources={ns}", flush=True)
     dev = torch.device("cuda")

@@ -170,6 +170,34 @@ int mbem_lu_factor(double *A, int64_t n, int64_t *ipiv, int64_t *info, hipStream
 int mbem_lu_solve(const double *LU, int64_t n, const int64_t *ipiv, double *b, int64_t nrhs,
                   hipStream_t stream);
 
+/* Blocked triangular solves on the factors of mbem_lu_factor (the solve of
+ * the reference's LU path, la/lu.py:180-239, and the condition estimate's
+ * solves): mbem_lu_prepare inverts the 64 x 64 diagonal blocks of L and U
+ * once per factorization into inv_l, inv_u (mbem_lu_inv_doubles(n) doubles
+ * each); mbem_lu_solve_blocked then solves A x = b (adjoint == 0) or
+ * A^T x = b (adjoint != 0), one launch per diagonal block, with perm the
+ * row order of the pivots (P^T v = v[perm], perm from ipiv by applying its
+ * interchanges in order to 0 .. n-1) and work 2 n doubles.  b and x may be
+ * the same vector. */
+int64_t mbem_lu_inv_doubles(int64_t n);
+int mbem_lu_prepare(const double *LU, int64_t n, double *inv_l, double *inv_u, hipStream_t stream);
+int mbem_lu_solve_blocked(const double *LU, int64_t n, const int64_t *perm, const double *inv_l,
+                          const double *inv_u, const double *b, double *x, double *work,
+                          int adjoint, hipStream_t stream);
+
+/* A^T x = b with the factors of mbem_lu_factor, in place, nrhs columns: the
+ * transposed solves of the 1-norm condition estimate that LAPACK's gecon
+ * makes on the reference's factors (la/lu.py:241-248). */
+int mbem_lu_solve_adjoint(const double *LU, int64_t n, const int64_t *ipiv, double *b,
+                          int64_t nrhs, hipStream_t stream);
+
+/* colsum[c] = sum_r |A[r, c]| for an (m, n) row-major block, summed in a
+ * fixed order through work (mbem_colsum_work(n) doubles): the 1-norm
+ * max_c colsum[c] of the factored matrix (la/lu.py:142-145 anorm). */
+int64_t mbem_colsum_work(int64_t n);
+int mbem_colsum_abs(const double *A, int64_t lda, int64_t m, int64_t n, double *work,
+                    double *colsum, hipStream_t stream);
+
 /* Block-Jacobi apply (la/preconditioner.py:456-462 + _bj_solve :601-630):
  * z[idx[c][i]] = sum_j inv_c[i,j] r[idx[c][j]] for every chunk c.  chunk_ptr
  * (nchunks+1) delimits each chunk's DOF list in idx, inv_ptr its (m_c, m_c)

@@ -51,6 +51,12 @@ _SIGS = {
     "mbem_copy2d": [_p, _i64, _p, _i64, _i64, _i64, _int, _p],
     "mbem_lu_factor": [_p, _i64, _p, _p, _p],
     "mbem_lu_solve": [_p, _i64, _p, _p, _i64, _p],
+    "mbem_lu_solve_adjoint": [_p, _i64, _p, _p, _i64, _p],
+    "mbem_lu_inv_doubles": [_i64],
+    "mbem_lu_prepare": [_p, _i64, _p, _p, _p],
+    "mbem_lu_solve_blocked": [_p, _i64, _p, _p, _p, _p, _p, _p, _int, _p],
+    "mbem_colsum_work": [_i64],
+    "mbem_colsum_abs": [_p, _i64, _i64, _i64, _p, _p, _p],
     "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
     "mbem_cast_lo": [_p, _i64, _i64, _i64, _p, _i64, _int, _p],
     "mbem_near_pattern": [_p, _p, _p, _i64, _p, _p, _p, _p],
@@ -110,7 +116,7 @@ _SIGS = {
     "mbem_lr_outer": [_p, _i64, _p, _i64, _i64, _i64, _i64, _p, _p],
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }
-_RESTYPE = {"mbem_multidot_work": _i64, "mbem_i8_segments": _i64}
+_RESTYPE = {"mbem_multidot_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
 
 _lib = None
 

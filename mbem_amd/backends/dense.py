@@ -143,6 +143,33 @@ def translation_basis(layout) -> np.ndarray:
     return Z
 
 
+class _LUFactors:
+    """Device LU factors of a row-major matrix M (mbem_lu_factor) with what
+    the blocked solves need: the inverses of the 64 x 64 diagonal blocks of
+    L and U and the pivots as a row order (P^T v = v[perm])."""
+
+    def __init__(self, LU: torch.Tensor, ipiv: torch.Tensor, m: int):
+        self.LU, self.ipiv, self.m = LU, ipiv, m
+        dev = LU.device
+        k = int(_lib.load().mbem_lu_inv_doubles(m))
+        self.inv_l = torch.empty(max(k, 1), dtype=F64, device=dev)
+        self.inv_u = torch.empty(max(k, 1), dtype=F64, device=dev)
+        call("mbem_lu_prepare", ptr(LU), m, ptr(self.inv_l), ptr(self.inv_u), stream())
+        piv = ipiv.cpu().numpy() - 1            # rocSOLVER pivots are 1-based
+        perm = np.arange(m, dtype=np.int64)
+        for i, p in enumerate(piv.tolist()):
+            if p != i:
+                perm[i], perm[p] = perm[p], perm[i]
+        self.perm = torch.as_tensor(perm, device=dev)
+        self.work = torch.empty(2 * max(m, 1), dtype=F64, device=dev)
+
+    def solve_into(self, b: torch.Tensor, x: torch.Tensor, adjoint: bool = False):
+        """x = M^-1 b (adjoint: M^-T b); b and x may be the same vector."""
+        call("mbem_lu_solve_blocked", ptr(self.LU), self.m, ptr(self.perm), ptr(self.inv_l),
+             ptr(self.inv_u), ptr(b), ptr(x), ptr(self.work), int(adjoint), stream())
+        return x
+
+
 @dataclass
 class DenseSolveReport:
     """dense.py:260-271 (LU solve)."""
@@ -486,20 +513,52 @@ class AssembledDense:
             ipiv = torch.empty(m, dtype=torch.int64, device=dev)
             info = torch.zeros(1, dtype=torch.int64, device=dev)
             t0 = time.perf_counter()
+            anorm = self._norm1(LU)
             call("mbem_lu_factor", ptr(LU), m, ptr(ipiv), ptr(info), stream())
             if int(info.item()) != 0:
                 raise np.linalg.LinAlgError(f"dense LU: singular (getrf info {int(info.item())})")
-            self._lu = (LU, ipiv, m, time.perf_counter() - t0)
-        LU, ipiv, m, fs = self._lu
+            fs = time.perf_counter() - t0
+            fac = _LUFactors(LU, ipiv, m)
+            self._lu = (fac, fs, self._cond1(fac, anorm))
+        fac, fs, cond = self._lu
+        m = fac.m
         rhs = torch.zeros(m, dtype=F64, device=dev)
         rhs[:n] = self.b_dev
-        call("mbem_lu_solve", ptr(LU), m, ptr(ipiv), ptr(rhs), 1, stream())
+        fac.solve_into(rhs, rhs)
         x = rhs[:n].contiguous()
         r = self.b_dev - self.matvec(x)
         res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(self.b_dev))
-        self.report = DenseSolveReport(cond_estimate=float("nan"),
+        self.report = DenseSolveReport(cond_estimate=cond,
                                        lu={"how": "double", "factor_s": fs}, residual=res)
         return x
+
+    @staticmethod
+    def _norm1(M: torch.Tensor) -> float:
+        """max_c sum_r |M[r, c]| of a square device matrix (la/lu.py:142-145)."""
+        m = M.shape[0]
+        colsum = torch.empty(m, dtype=F64, device=M.device)
+        work = torch.empty(int(_lib.load().mbem_colsum_work(m)), dtype=F64, device=M.device)
+        call("mbem_colsum_abs", ptr(M), M.stride(0), m, m, ptr(work), ptr(colsum), stream())
+        return float(colsum.max().item()) if m else 0.0
+
+    @staticmethod
+    def _cond1(fac: "_LUFactors", anorm: float) -> float:
+        """1-norm condition estimate of the factored matrix: gecon's
+        rcond = (1 / est ||M^-1||_1) / ||M||_1 from the device solves, then
+        1 / rcond, infinite when rcond is 0 (la/lu.py:241-248)."""
+        from ..la.condest import onenorm_inverse
+        v = torch.empty(fac.m, dtype=F64, device=fac.LU.device)
+
+        def solve(x, adjoint):
+            v.copy_(torch.from_numpy(np.ascontiguousarray(x, dtype=np.float64)))
+            fac.solve_into(v, v, adjoint=adjoint)
+            return v.cpu().numpy()
+
+        ainv = onenorm_inverse(lambda x: solve(x, False), lambda x: solve(x, True), fac.m)
+        if ainv == 0.0 or anorm == 0.0:
+            return float("inf")
+        rcond = (1.0 / ainv) / anorm
+        return (1.0 / rcond) if (np.isfinite(rcond) and rcond > 0) else float("inf")
 
     def release(self):
         """Drop the factors and the preconditioner (dense.py:573-576)."""

@@ -143,6 +143,35 @@ __global__ __launch_bounds__(256) void k_normalize(long n, const double* __restr
     if (nrm > 1e-300) w[j] = w[j] / nrm;
 }
 
+// part[p, c] = sum over rows r of chunk p of |A[r, c]| (a thread per column,
+// the wave's columns contiguous), then colsum[c] = sum_p part[p, c]: the
+// column sums in a fixed order, so the 1-norm is reproducible
+constexpr int kNormChunks = 64;
+
+__global__ __launch_bounds__(256) void k_colsum_abs_part(const double* __restrict__ A, long lda,
+                                                         long m, long n,
+                                                         double* __restrict__ part)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const long rows = (m + kNormChunks - 1) / kNormChunks;
+    const long r0 = blockIdx.y * rows;
+    const long r1 = (r0 + rows < m) ? r0 + rows : m;
+    double s = 0.0;
+    for (long r = r0; r < r1; ++r) s += fabs(A[r * lda + c]);
+    part[blockIdx.y * n + c] = s;
+}
+
+__global__ __launch_bounds__(256) void k_colsum_reduce(const double* __restrict__ part, long n,
+                                                       double* __restrict__ colsum)
+{
+    const long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double s = 0.0;
+    for (int p = 0; p < kNormChunks; ++p) s += part[p * n + c];
+    colsum[c] = s;
+}
+
 // chunk gather: M[i, j] = A[idx[i], idx[j]]
 __global__ __launch_bounds__(256) void k_gather_block(const double* __restrict__ A, long lda,
                                                       const int64_t* __restrict__ idx, long m,
@@ -218,6 +247,140 @@ __global__ __launch_bounds__(256) void k_bj_apply(const int64_t* __restrict__ wt
         long i = row0 + q;
         if (lane == 0 && i < m) z[id[i]] = t;
     }
+}
+
+
+// ---- blocked triangular solves on the LU factors ---------------------------
+// The row-major buffer that mbem_lu_factor factors is A^T in rocSOLVER's
+// column-major view: A^T = P L U with L (unit lower) and U stored column-major
+// (element (i, j) at LU[i + j ld]).  A x = b is U^T L^T P^T x = b: a forward
+// solve with U^T and a backward one with L^T, both read row-wise (element
+// (i, j) of U^T at LU[j + i ld]), then x = P z.  A^T x = b is P L U x = b:
+// z = P^T b, then L forward and U backward, read column-wise.  Each solve
+// walks kTB-row diagonal blocks; the diagonal blocks' inverses are formed
+// once per factorization, so one step is one launch: every workgroup forms
+// x_K = inv(T_KK) r_K itself and updates its own rows beyond K.
+constexpr int kTB = 64;
+
+// inverse of diagonal block blockIdx.x of L (LOWER, unit) or U, row-major
+// per block: inv[(k kTB + i) kTB + j]; a partial last block is padded with
+// the identity.  Thread t forms column t by substitution.
+template <bool LOWER>
+__global__ __launch_bounds__(kTB) void k_tri_inv_diag(const double* __restrict__ LU, long ld, long n,
+                                                      double* __restrict__ inv)
+{
+    __shared__ double T[kTB][kTB + 1];
+    __shared__ double Y[kTB][kTB + 1];
+    const int t = threadIdx.x;
+    const long k0 = (long)blockIdx.x * kTB;
+    const int nb = (n - k0 < kTB) ? (int)(n - k0) : kTB;
+    for (int i = 0; i < kTB; ++i)
+        T[i][t] = (i < nb && t < nb) ? LU[(k0 + i) + (k0 + t) * ld] : (i == t ? 1.0 : 0.0);
+    __syncthreads();
+    if (LOWER) {
+        for (int i = 0; i < kTB; ++i) {
+            double s = (i == t) ? 1.0 : 0.0;
+            for (int j = 0; j < i; ++j) s -= T[i][j] * Y[j][t];
+            Y[i][t] = s;
+        }
+    } else {
+        for (int i = kTB - 1; i >= 0; --i) {
+            double s = (i == t) ? 1.0 : 0.0;
+            for (int j = i + 1; j < kTB; ++j) s -= T[i][j] * Y[j][t];
+            Y[i][t] = s / T[i][i];
+        }
+    }
+    for (int i = 0; i < kTB; ++i) inv[((long)blockIdx.x * kTB + i) * kTB + t] = Y[i][t];
+}
+
+// one step over diagonal block k of T x = r.  ROWWISE: T(i, j) = LU[j + i ld]
+// (U^T or L^T) and inv(T_KK) is the stored block inverse transposed; else
+// T(i, j) = LU[i + j ld].  LOWER: the rows after K are updated (forward),
+// else the rows before K (backward).  Workgroup 0 writes x_K; the updated
+// rows are each owned by one workgroup.
+template <bool LOWER, bool ROWWISE>
+__global__ __launch_bounds__(256) void k_trsv_step(const double* __restrict__ LU, long ld, long n,
+                                                   const double* __restrict__ inv, long k,
+                                                   double* __restrict__ r, double* __restrict__ x)
+{
+    __shared__ double xk[kTB];
+    const int t = threadIdx.x;
+    const long k0 = k * kTB;
+    const int nb = (n - k0 < kTB) ? (int)(n - k0) : kTB;
+    {
+        // x_K[i] = sum_j inv(T_KK)[i, j] r[k0 + j]: 4 lanes per row, 16 j each
+        const int i = t >> 2, q = t & 3;
+        const double* blk = inv + k * kTB * kTB;
+        double s = 0.0;
+#pragma unroll 4
+        for (int jj = 0; jj < 16; ++jj) {
+            const int j = q * 16 + jj;
+            const double rj = (j < nb) ? r[k0 + j] : 0.0;
+            s += (ROWWISE ? blk[j * kTB + i] : blk[i * kTB + j]) * rj;
+        }
+        s += __shfl_xor(s, 1, 64);
+        s += __shfl_xor(s, 2, 64);
+        if (q == 0) xk[i] = s;
+    }
+    __syncthreads();
+    if (blockIdx.x == 0 && t < nb) x[k0 + t] = xk[t];
+    // rows this workgroup updates
+    const long lo = LOWER ? k0 + nb : 0;
+    const long hi = LOWER ? n : k0;
+    if (!ROWWISE) {
+        const long i = lo + (long)blockIdx.x * 256 + t;
+        if (i >= hi) return;
+        const double* col = LU + i + k0 * ld;
+        double s = 0.0;
+        for (int j = 0; j < nb; ++j) s += col[j * ld] * xk[j];
+        r[i] -= s;
+    } else {
+        // 16 lanes per row, 4 consecutive entries a lane; 64 rows a workgroup
+        const int g = t >> 4, l = t & 15;
+        for (int pass = 0; pass < 4; ++pass) {
+            const long i = lo + (long)blockIdx.x * 64 + pass * 16 + g;
+            double s = 0.0;
+            if (i < hi) {
+                const double* row = LU + i * ld + k0;
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int j = 4 * l + e;
+                    if (j < nb) s += row[j] * xk[j];
+                }
+            }
+#pragma unroll
+            for (int off = 8; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (l == 0 && i < hi) r[i] -= s;
+        }
+    }
+}
+
+// out[i] = v[perm[i]] (GATHER) or out[perm[i]] = v[i]
+template <bool GATHER>
+__global__ __launch_bounds__(256) void k_permute_vec(const double* __restrict__ v,
+                                                     const int64_t* __restrict__ perm, long n,
+                                                     double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    if (GATHER) out[i] = v[perm[i]];
+    else out[perm[i]] = v[i];
+}
+
+template <bool LOWER, bool ROWWISE>
+int trsv_blocked(const double* LU, long n, const double* inv, double* r, double* x, hipStream_t st)
+{
+    const long nblk = (n + kTB - 1) / kTB;
+    for (long q = 0; q < nblk; ++q) {
+        const long k = LOWER ? q : nblk - 1 - q;
+        const long k0 = k * kTB;
+        const long nb = (n - k0 < kTB) ? n - k0 : kTB;
+        const long rows = LOWER ? n - k0 - nb : k0;
+        const long per = ROWWISE ? 64 : 256;
+        const long g = rows > 0 ? (rows + per - 1) / per : 1;
+        hipLaunchKernelGGL((k_trsv_step<LOWER, ROWWISE>), dim3((unsigned)g), dim3(256), 0, st, LU, n, n, inv, k, r, x);
+    }
+    return hipGetLastError() == hipSuccess ? MBEM_OK : MBEM_EHIP;
 }
 
 rocblas_handle g_handle = nullptr;
@@ -390,6 +553,72 @@ int mbem_lu_solve(const double* LU, int64_t n, const int64_t* ipiv, double* b, i
     if (rocsolver_dgetrs_64(g_handle, rocblas_operation_transpose, n, nrhs, const_cast<double*>(LU), n,
                             const_cast<int64_t*>(ipiv), b, n) != rocblas_status_success)
         return MBEM_EHIP;
+    return launch_status();
+}
+
+int mbem_lu_solve_adjoint(const double* LU, int64_t n, const int64_t* ipiv, double* b,
+                          int64_t nrhs, hipStream_t stream)
+{
+    if (n < 0 || nrhs < 0 || (n > 0 && (!LU || !ipiv || !b))) return MBEM_EINVAL;
+    if (n == 0 || nrhs == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    // the row-major factors are those of A^T in rocSOLVER's column-major view,
+    // so A^T x = b is the untransposed solve
+    if (rocsolver_dgetrs_64(g_handle, rocblas_operation_none, n, nrhs, const_cast<double*>(LU), n,
+                            const_cast<int64_t*>(ipiv), b, n) != rocblas_status_success)
+        return MBEM_EHIP;
+    return launch_status();
+}
+
+int64_t mbem_colsum_work(int64_t n) { return n <= 0 ? 0 : kNormChunks * n; }
+
+int mbem_colsum_abs(const double* A, int64_t lda, int64_t m, int64_t n, double* work,
+                    double* colsum, hipStream_t stream)
+{
+    if (m < 0 || n < 0 || (n > 0 && (!A || !work || !colsum || lda < n))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    dim3 grid((unsigned)((n + 255) / 256), kNormChunks), block(256);
+    hipLaunchKernelGGL(k_colsum_abs_part, grid, block, 0, stream, A, lda, m, n, work);
+    hipLaunchKernelGGL(k_colsum_reduce, dim3((unsigned)((n + 255) / 256)), block, 0, stream, work, n, colsum);
+    return launch_status();
+}
+
+int64_t mbem_lu_inv_doubles(int64_t n) { return n <= 0 ? 0 : ((n + kTB - 1) / kTB) * kTB * kTB; }
+
+int mbem_lu_prepare(const double* LU, int64_t n, double* inv_l, double* inv_u, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!LU || !inv_l || !inv_u))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    const long nblk = (n + kTB - 1) / kTB;
+    hipLaunchKernelGGL((k_tri_inv_diag<true>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_l);
+    hipLaunchKernelGGL((k_tri_inv_diag<false>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_u);
+    return launch_status();
+}
+
+int mbem_lu_solve_blocked(const double* LU, int64_t n, const int64_t* perm, const double* inv_l,
+                          const double* inv_u, const double* b, double* x, double* work,
+                          int adjoint, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!LU || !perm || !inv_l || !inv_u || !b || !x || !work))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    double* r = work;
+    double* y = work + n;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    int rc;
+    if (!adjoint) {
+        // U^T w = b, L^T z = w, x = P z
+        if (hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            return MBEM_EHIP;
+        if ((rc = trsv_blocked<true, true>(LU, n, inv_u, r, y, stream)) != MBEM_OK) return rc;
+        if ((rc = trsv_blocked<false, true>(LU, n, inv_l, y, r, stream)) != MBEM_OK) return rc;
+        hipLaunchKernelGGL((k_permute_vec<false>), grid, block, 0, stream, r, perm, n, x);
+    } else {
+        // z = P^T b, L y = z, U x = y
+        hipLaunchKernelGGL((k_permute_vec<true>), grid, block, 0, stream, b, perm, n, r);
+        if ((rc = trsv_blocked<true, false>(LU, n, inv_l, r, y, stream)) != MBEM_OK) return rc;
+        if ((rc = trsv_blocked<false, false>(LU, n, inv_u, y, x, stream)) != MBEM_OK) return rc;
+    }
     return launch_status();
 }
 

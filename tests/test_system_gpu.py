@@ -6,6 +6,9 @@ Bars (FP64, north star 1e-10 relative):
   A, b, calibrated diagonal : every entry within 1e-10 of max|.|, Frobenius
                               within 1e-12;
   LU solution               : within 1e-10 (relative 2-norm) of the oracle's LU;
+  LU condition estimate     : within 1e-6 of LAPACK gecon's on the oracle's
+                              matrix (the same estimator on other factors of
+                              the same matrix);
   FGMRES solution           : converged to true relres 1e-13, then within
                               1e-10 of the oracle's LU, iteration count
                               within 2 of the oracle's MGS FGMRES;
@@ -67,6 +70,15 @@ def test_lu_solve(c1):
     x = np.concatenate([sol[s.name].ravel() for s in asm.layout.slots])
     print(f"LU: rel err vs oracle LU {_rel(x, c1['x_lu']):.2e}, residual {asm.report.residual:.2e}")
     assert _rel(x, c1["x_lu"]) < 1e-10
+    import scipy.linalg as sl
+    from scipy.linalg.lapack import dgecon
+    A = c1["A"]
+    lu, _ = sl.lu_factor(A)
+    rcond, info = dgecon(lu, float(np.abs(A).sum(axis=0).max()), norm="1")
+    ref = 1.0 / rcond
+    got = asm.report.cond_estimate
+    print(f"LU cond estimate {got:.6e}, LAPACK gecon {ref:.6e}", flush=True)
+    assert info == 0 and abs(got - ref) <= 1e-6 * ref
     asm.release()
 
 
