@@ -86,42 +86,69 @@ __global__ __launch_bounds__(256) void k_gemv(const double* __restrict__ A, long
 }
 
 // split-K partial dots: part[chunk * k + row] = sum_{j in chunk} V[row, j] w[j]
+// part[c, row] = sum over columns of chunk c (kDotChunk wide) of V[row, j] w[j]:
+// a wave per (row, chunk), 16-byte loads when rows and w are 16-byte aligned
+constexpr long kDotChunk = 512;
+
 __global__ __launch_bounds__(256) void k_multidot_part(const double* __restrict__ V, long ldv,
                                                        long k, long n, const double* __restrict__ w,
-                                                       long chunk, double* __restrict__ part)
+                                                       double* __restrict__ part, int vec)
 {
     const int lane = threadIdx.x & 63;
     const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const long c = blockIdx.y;
     if (row >= k) return;
-    const long j0 = c * chunk, j1 = (j0 + chunk < n) ? j0 + chunk : n;
-    double acc = 0.0;
+    const long j0 = c * kDotChunk;
     const double* v = V + row * ldv;
-    for (long j = j0 + lane; j < j1; j += 64) acc += v[j] * w[j];
-    acc = wave_sum(acc);
+    double a0 = 0.0, a1 = 0.0;
+    if (vec && j0 + kDotChunk <= n) {
+#pragma unroll
+        for (int s = 0; s < (int)(kDotChunk / 128); ++s) {
+            const long j = j0 + s * 128 + 2 * lane;
+            const dbl2 vv = *reinterpret_cast<const dbl2*>(v + j);
+            const dbl2 ww = *reinterpret_cast<const dbl2*>(w + j);
+            a0 += vv.x * ww.x;
+            a1 += vv.y * ww.y;
+        }
+    } else {
+        const long j1 = (j0 + kDotChunk < n) ? j0 + kDotChunk : n;
+        for (long j = j0 + lane; j < j1; j += 64) a0 += v[j] * w[j];
+    }
+    const double acc = wave_sum(a0 + a1);
     if (lane == 0) part[c * k + row] = acc;
 }
 
+// h[row] = sum_c part[c, row]: a wave per row, lanes over the parts
 __global__ __launch_bounds__(256) void k_reduce_parts(const double* __restrict__ part, long k,
                                                       long nparts, double* __restrict__ h)
 {
-    long row = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (row >= k) return;
     double s = 0.0;
-    for (long c = 0; c < nparts; ++c) s += part[c * k + row];
-    h[row] = s;
+    for (long c = lane; c < nparts; c += 64) s += part[c * k + row];
+    s = wave_sum(s);
+    if (lane == 0) h[row] = s;
 }
 
-// w[j] += alpha * sum_row V[row, j] h[row]
+// w[j] += alpha * sum_row V[row, j] h[row]: a workgroup per 64 columns, its
+// four waves taking every fourth row, the partial sums combined in LDS in a
+// fixed order
 __global__ __launch_bounds__(256) void k_multiaxpy(const double* __restrict__ V, long ldv, long k,
                                                    long n, const double* __restrict__ h,
                                                    double* __restrict__ w, double alpha)
 {
-    long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= n) return;
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long j = (long)blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (long row = 0; row < k; ++row) s += V[row * ldv + j] * h[row];
-    w[j] += alpha * s;
+    if (j < n) {
+#pragma unroll 4
+        for (long row = wv; row < k; row += 4) s += V[row * ldv + j] * h[row];
+    }
+    red[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && j < n) w[j] += alpha * ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
 }
 
 __global__ __launch_bounds__(256) void k_axpby(long n, double a, const double* __restrict__ x,
@@ -408,8 +435,8 @@ int mbem_gemv(const double* A, int64_t lda, int64_t m, int64_t n, const double* 
 
 int64_t mbem_multidot_work(int64_t k, int64_t n)
 {
-    const int64_t chunk = 4096;
-    int64_t parts = (n + chunk - 1) / chunk;
+    int64_t parts = (n + kDotChunk - 1) / kDotChunk;
+    if (parts < 1) parts = 1;
     return parts * (k > 0 ? k : 1);
 }
 
@@ -418,12 +445,14 @@ int mbem_multidot(const double* V, int64_t ldv, int64_t k, int64_t n, const doub
 {
     if (k < 0 || n < 0 || (k > 0 && (!V || !h || !work || ldv < n)) || (n > 0 && !w)) return MBEM_EINVAL;
     if (k == 0) return MBEM_OK;
-    const int64_t chunk = 4096;
-    int64_t parts = (n + chunk - 1) / chunk;
+    int64_t parts = (n + kDotChunk - 1) / kDotChunk;
     if (parts == 0) parts = 1;
+    if (parts > 65535) return MBEM_EINVAL;      // grid.y
+    const int vec = (ldv % 2 == 0) && (reinterpret_cast<uintptr_t>(V) % 16 == 0) &&
+                    (reinterpret_cast<uintptr_t>(w) % 16 == 0);
     dim3 grid((unsigned)((k + 3) / 4), (unsigned)parts), block(256);
-    hipLaunchKernelGGL(k_multidot_part, grid, block, 0, stream, V, ldv, k, n, w, chunk, work);
-    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((k + 255) / 256)), dim3(256), 0, stream, work, k, parts, h);
+    hipLaunchKernelGGL(k_multidot_part, grid, block, 0, stream, V, ldv, k, n, w, work, vec);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, stream, work, k, parts, h);
     return launch_status();
 }
 
@@ -432,7 +461,7 @@ int mbem_multiaxpy(const double* V, int64_t ldv, int64_t k, int64_t n, const dou
 {
     if (k < 0 || n < 0 || (k > 0 && n > 0 && (!V || !h || !w || ldv < n))) return MBEM_EINVAL;
     if (k == 0 || n == 0) return MBEM_OK;
-    hipLaunchKernelGGL(k_multiaxpy, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, V, ldv, k, n, h, w, alpha);
+    hipLaunchKernelGGL(k_multiaxpy, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, stream, V, ldv, k, n, h, w, alpha);
     return launch_status();
 }
 
