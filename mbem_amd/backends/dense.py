@@ -23,6 +23,7 @@ every C_q as the reference does (the skipped ones never reach A or b).
 """
 from __future__ import annotations
 
+import math
 import time
 import warnings
 from dataclasses import dataclass
@@ -143,6 +144,35 @@ def translation_basis(layout) -> np.ndarray:
     return Z
 
 
+def material_step(old: dict, new: dict) -> float:
+    """max |log(new/old)| over the moduli of the regions that changed; infinite
+    when a region appears or a modulus changes sign (hmat.py:33-52)."""
+    step = 0.0
+    for name, mat in new.items():
+        prev = old.get(name)
+        if prev is None:
+            return math.inf
+        for a, b in ((mat.mu, prev.mu), (mat.lam, prev.lam)):
+            if a == b:
+                continue
+            if a <= 0.0 or b <= 0.0:
+                return math.inf
+            step = max(step, abs(math.log(a / b)))
+    return step
+
+
+class _Lineage:
+    """What one chain of ``rebuild_for_materials`` shares across its solves
+    (hmat.py:55-70): the last solution (the warm start of
+    ``x0="previous"``) and the recycle space with the materials it was last
+    harvested at."""
+
+    def __init__(self):
+        self.x = None
+        self.recycle = None
+        self.recycle_materials = None
+
+
 class _LUFactors:
     """Device LU factors of a row-major matrix M (mbem_lu_factor) with what
     the blocked solves need: the inverses of the 64 x 64 diagonal blocks of
@@ -224,6 +254,7 @@ class AssembledDense:
         self._precond = None
         self._precond_caps = None
         self._lu = None
+        self._lineage = _Lineage()
         self._A_host = None
         self._b_host = None
         self.A_dev = None
@@ -429,7 +460,37 @@ class AssembledDense:
         FGMRES, DenseSolveReport for LU).  ``precond_grouping`` ("patch": a
         Gauss-Seidel sweep over patches, the reference's default; "region":
         one super-block per region, i.e. plain block-Jacobi over the cluster
-        chunks) is an addition to the reference's keywords."""
+        chunks) is an addition to the reference's keywords.
+
+        The FGMRES route takes the compressed backend's two sequence options
+        for the solves of one ``rebuild_for_materials`` chain
+        (hmat.py:605-664): ``x0="previous"`` warm-starts from the chain's last
+        solution, and ``recycle=True`` (default
+        ``defaults.GCRO_RECYCLE_DEFAULT``) carries a GCRO-DR subspace through
+        it, dropped when the materials have moved more than
+        ``defaults.PRECOND_REUSE_MAX_STEP`` since it was harvested.  A
+        ``RecycleSpace`` may also be passed to carry one explicitly."""
+        lineage = self._lineage
+        if isinstance(x0, str):
+            if x0 != "previous":
+                raise ValueError(f"x0 must be an array, None or 'previous' (got {x0!r})")
+            x0 = lineage.x if (lineage.x is not None
+                               and lineage.x.shape[0] == self.layout.n_unknowns) else None
+        rs = None
+        if method == "fgmres":
+            from ..la.solver import RecycleSpace
+            if isinstance(recycle, RecycleSpace):
+                rs = recycle
+            elif defaults.GCRO_RECYCLE_DEFAULT if recycle is None else recycle:
+                if lineage.recycle is None:
+                    lineage.recycle = RecycleSpace()
+                elif lineage.recycle_materials is not None and material_step(
+                        lineage.recycle_materials, self.materials) > \
+                        defaults.PRECOND_REUSE_MAX_STEP:
+                    lineage.recycle.reset()
+                rs = lineage.recycle
+                lineage.recycle_materials = dict(self.materials)
+        recycle = rs
         if method == "lu":
             x = self._solve_lu()
         elif method == "fgmres":
@@ -439,6 +500,7 @@ class AssembledDense:
         else:
             raise ValueError(f"solve method {method!r} (expected 'fgmres' or 'lu')")
         self.x_dev = x
+        lineage.x = x.detach().clone()
         if return_device:
             return x
         xh = x.cpu().numpy()
@@ -573,9 +635,11 @@ class AssembledDense:
             if name not in mats:
                 raise KeyError(f"unknown region '{name}'")
             mats[name] = mat
-        return AssembledDense(self.system, self.eps, self.mode, jump=self.jump,
-                              deflate=self.deflate, lu=self.lu,
-                              calibrate_all_rows=self.calibrate_all_rows, _materials=mats)
+        new = AssembledDense(self.system, self.eps, self.mode, jump=self.jump,
+                             deflate=self.deflate, lu=self.lu,
+                             calibrate_all_rows=self.calibrate_all_rows, _materials=mats)
+        new._lineage = self._lineage            # one chain, one layout
+        return new
 
 
 class _LazyCalib(dict):

@@ -14,6 +14,9 @@ MAX_DENSE_PRECOND_DOF = 30_000     # defaults.py:645
 PRECOND_DENSE_RAM_FRACTION = 0.06  # defaults.py:653
 PRECOND_RUNG_ABOVE_DENSE = "block_jacobi"  # defaults.py:695
 PRECOND_BJ_CHUNK_DOF = 9000        # defaults.py:712
+PRECOND_REUSE_MAX_STEP = 0.6931471805599453   # defaults.py:78 (= log 2)
+GCRO_RECYCLE_DIM = 30              # defaults.py:94
+GCRO_RECYCLE_DEFAULT = False       # defaults.py:99
 EPS_OVER_H = 0.1                   # defaults.py:798
 FAULT_EPS_OVER_H = 0.07            # defaults.py:804
 HALF_JUMP_MAX_EPS_OVER_H = 0.5     # defaults.py:808

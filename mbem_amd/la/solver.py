@@ -62,6 +62,43 @@ class SolveReport:
         return s
 
 
+class RecycleSpace:
+    """The subspace GCRO-DR carries from one solve of a sequence to the next
+    (solver.py:72-112): ``vectors`` (n x k') in the domain of the
+    preconditioned operator A M, with unit columns; each solve re-derives
+    U = M Y and C = A U for its own A and M, so both may change between
+    solves.  Held on the device as k' rows; ``reset()`` drops them (a changed
+    layout or a material step past the reuse window)."""
+
+    def __init__(self, dim: int = defaults.GCRO_RECYCLE_DIM):
+        self.dim = int(dim)
+        self._rows = None
+
+    @property
+    def vectors(self):
+        return None if self._rows is None else self._rows.T
+
+    @vectors.setter
+    def vectors(self, Y):
+        if Y is None:
+            self._rows = None
+            return
+        Y = torch.as_tensor(Y, dtype=F64, device=_lib.device())
+        self._store(Y.T.contiguous())
+
+    def reset(self):
+        self._rows = None
+
+    def _store(self, rows):
+        """Keep the span with unit rows (zero rows dropped), as a copy."""
+        if rows is None or rows.numel() == 0:
+            self._rows = None
+            return
+        norms = torch.linalg.vector_norm(rows, dim=1)
+        keep = norms > 0.0
+        self._rows = (rows[keep] / norms[keep, None]).contiguous() if bool(keep.any()) else None
+
+
 def _solve_upper(R, g):
     """Back substitution of the small upper-triangular system (solver.py:503-510)."""
     m = R.shape[0]
@@ -157,8 +194,6 @@ def fgmres(A, b, M=None, x0=None,
 
     ``A`` and ``M`` are device matrices or callables ``f(v) -> A v`` on
     device vectors; ``M`` approximates A^-1 (right preconditioner)."""
-    if recycle is not None:
-        raise NotImplementedError("GCRO-DR recycling is not in this build")
     _lib.require_gpu()
     A_mv = _as_op(A)
     M_mv = _as_op(M) if M is not None else None
@@ -172,6 +207,13 @@ def fgmres(A, b, M=None, x0=None,
         return torch.zeros(n, dtype=F64, device=dev), SolveReport(True, 0, 0.0, 0.0)
     x = torch.zeros(n, dtype=F64, device=dev) if x0 is None else \
         torch.as_tensor(x0, dtype=F64, device=dev).clone().contiguous()
+    if recycle is not None:
+        # GCRO-DR with the subspace carried in ``recycle`` (solver.py:248-395)
+        from .gcrodr import GcroDR
+        g = GcroDR(n, max(2, restart), recycle.dim, dev)
+        return g.solve(A_mv, M_mv if M_mv is not None else (lambda v, out: out.copy_(v)), b, x,
+                       x0 is None, recycle, rtol, maxiter, stagnation_window, stagnation_factor,
+                       callback)
 
     history: list = []
     total_iters = 0

@@ -246,3 +246,151 @@ class BlockJacobiGS:
             for dof, lu in chunks:
                 z[dof] = sla.lu_solve(lu, full[dof])
         return z
+
+
+def _gcro_lstsq(G, rhs):
+    y, *_ = np.linalg.lstsq(G, rhs, rcond=None)
+    return y, float(np.linalg.norm(rhs - G @ y))
+
+
+def _gcro_G(d, B, Hbar):
+    """[[diag(d), B], [0, Hbar]] (la/solver.py:398-407)."""
+    k = 0 if d is None else len(d)
+    G = np.zeros((k + Hbar.shape[0], k + Hbar.shape[1]))
+    if k:
+        G[:k, :k] = np.diag(d)
+        G[:k, k:] = B
+    G[k:, k:] = Hbar
+    return G
+
+
+def harmonic_ritz_span(G, WtV, keep):
+    """Real orthonormal basis of the k smallest-|theta| eigenvectors of the
+    pencil G^T G z = theta G^T WtV z (la/solver.py:464-492)."""
+    import scipy.linalg as sl
+    theta, vec = sl.eig(G.T @ G, G.T @ WtV)
+    mag = np.where(np.isfinite(np.abs(theta)), np.abs(theta), np.inf)
+    idx = np.argsort(mag)[:keep]
+    if idx.size == 0 or not np.isfinite(mag[idx[0]]):
+        return None
+    sel = vec[:, idx]
+    parts = [sel.real] + ([sel.imag] if np.abs(sel.imag).max() > 0 else [])
+    Q, s, _ = np.linalg.svd(np.hstack(parts), full_matrices=False)
+    rank = int((s > 1e-12 * s[0]).sum()) if s.size else 0
+    if rank == 0:
+        return None
+    return Q[:, :min(rank, keep + 1, G.shape[1])]
+
+
+def gcrodr(A_mv, b, M=None, x0=None, Y=None, dim=30, rtol=1e-8, restart=200, maxiter=600,
+           stagnation_window=100, stagnation_factor=10.0):
+    """la/solver.py:248-395: GCRO-DR, right-preconditioned and flexible,
+    modified Gram-Schmidt Arnoldi.  ``Y`` (n x k) is the recycled space in
+    the A M domain; returns (x, info) with info["Y"] the space to carry."""
+    import scipy.linalg as sl
+    M_mv = M if M is not None else (lambda v: v)
+    b = np.asarray(b, dtype=float)
+    n = b.shape[0]
+    bn = np.linalg.norm(b)
+    x = np.zeros(n) if x0 is None else np.array(x0, dtype=float)
+    C = U = None
+    setup = 0
+    if Y is not None and Y.ndim == 2 and Y.shape[0] == n and Y.shape[1] > 0:
+        U0 = np.column_stack([M_mv(Y[:, i]) for i in range(Y.shape[1])])
+        AU = np.column_stack([A_mv(U0[:, i]) for i in range(Y.shape[1])])
+        setup = Y.shape[1]
+        Q, R = np.linalg.qr(AU)
+        dg = np.abs(np.diag(R))
+        if dg.min() > 1e-12 * max(dg.max(), 1e-300):
+            C = Q
+            Rinv = sl.solve_triangular(R, np.eye(R.shape[0]))
+            U, Y = U0 @ Rinv, Y @ Rinv
+        else:
+            Y = None
+    else:
+        Y = None
+    r = b - A_mv(x)
+    if C is not None:
+        c = C.T @ r
+        x = x + U @ c
+        r = r - C @ c
+    hist, total, cycles = [], 0, 0
+    best, best_at, stag = np.inf, 0, False
+    res = np.linalg.norm(r) / bn
+    while res >= rtol and total < maxiter and not stag:
+        k = 0 if C is None else C.shape[1]
+        m = min(restart - k, maxiter - total)
+        if m < 1:
+            break
+        beta = np.linalg.norm(r)
+        V = np.zeros((n, m + 1))
+        Z = np.zeros((n, m))
+        Hb = np.zeros((m + 1, m))
+        B = np.zeros((k, m))
+        V[:, 0] = r / beta
+        d = 1.0 / np.linalg.norm(Y, axis=0) if k else None
+        head = C.T @ r if k else np.zeros(0)
+        jd = 0
+        for j in range(m):
+            Z[:, j] = M_mv(V[:, j])
+            w = A_mv(Z[:, j])
+            if k:
+                B[:, j] = C.T @ w
+                w = w - C @ B[:, j]
+            for i in range(j + 1):
+                Hb[i, j] = V[:, i] @ w
+                w = w - Hb[i, j] * V[:, i]
+            Hb[j + 1, j] = np.linalg.norm(w)
+            V[:, j + 1] = w / Hb[j + 1, j] if Hb[j + 1, j] > 1e-300 else 0.0
+            total += 1
+            jd = j + 1
+            G = _gcro_G(d, B[:, :jd], Hb[:jd + 1, :jd])
+            _, ra = _gcro_lstsq(G, np.concatenate([head, [beta], np.zeros(jd)]))
+            res = ra / bn
+            hist.append(res)
+            if res < best / stagnation_factor:
+                best, best_at = res, total
+            elif total - best_at >= stagnation_window:
+                stag = True
+                break
+            if res < rtol or total >= maxiter:
+                break
+        if jd == 0:
+            break
+        G = _gcro_G(d, B[:, :jd], Hb[:jd + 1, :jd])
+        rhs = np.concatenate([head, [beta], np.zeros(jd)])
+        y, _ = _gcro_lstsq(G, rhs)
+        x = x + Z[:, :jd] @ y[k:]
+        if k:
+            x = x + (U * d) @ y[:k]
+        g = rhs - G @ y
+        r = V[:, :jd + 1] @ g[k:] + (C @ g[:k] if k else 0.0)
+        cycles += 1
+        if stag:
+            break
+        # the next space: harmonic Ritz vectors of the augmented pencil
+        Yh = Y * d if k else None
+        Uh = U * d if k else None
+        W = np.zeros((k + jd + 1, k + jd))
+        W[k:k + jd, k:] = np.eye(jd)
+        if k:
+            W[:k, :k] = C.T @ Yh
+            W[k:, :k] = V[:, :jd + 1].T @ Yh
+        P = harmonic_ritz_span(G, W, min(dim, k + jd))
+        if P is None:
+            C = U = Y = None
+            continue
+        Yn = V[:, :jd] @ P[k:] + (Yh @ P[:k] if k else 0.0)
+        Un = Z[:, :jd] @ P[k:] + (Uh @ P[:k] if k else 0.0)
+        Q, R = np.linalg.qr(G @ P)
+        dg = np.abs(np.diag(R))
+        if dg.size == 0 or dg.min() <= 1e-12 * max(dg.max(), 1e-300):
+            C = U = Y = None
+            continue
+        Cn = V[:, :jd + 1] @ Q[k:] + (C @ Q[:k] if k else 0.0)
+        Rinv = sl.solve_triangular(R, np.eye(R.shape[0]))
+        C, U, Y = Cn, Un @ Rinv, Yn @ Rinv
+    tr = float(np.linalg.norm(b - A_mv(x)) / bn)
+    Yout = None if Y is None else Y / np.linalg.norm(Y, axis=0)
+    return x, {"converged": tr < rtol, "iterations": total, "true_relres": tr, "history": hist,
+               "Y": Yout, "setup_matvecs": setup, "cycles": cycles}
