@@ -29,5 +29,4 @@ ASSEMBLY_SOURCE_CHUNK = 256        # defaults.py:981
 DENSE_LU = "double"                # defaults.py:1243
 LAPLACE_POINTS_PER_DECADE = 3      # defaults.py:1035
 LAPLACE_PAD_DECADES = 0.5          # defaults.py:1040
-LAPLACE_POINTS_PER_DECADE = 3      # defaults.py:1035
-LAPLACE_PAD_DECADES = 0.5          # defaults.py:1040
+PRONY_RCOND = 1.0e-13              # defaults.py:1067

@@ -244,3 +244,27 @@ def test_batched_states_match_per_state_lu_and_oracle(box):
             print(f"  oracle LU err {e_or:.2e} (bar {tol:.1e})", flush=True)
             assert e_or < tol
         del asm
+
+
+def test_time_map_of_the_batched_stack(box):
+    """laplace.py:119-312 on the device stack: the fit's weights applied by
+    the library GEMM equal numpy's product to 1e-13, and u(0+) is the elastic
+    state (the fit's first row pins it) to 1e-9."""
+    from mbem_amd import laplace
+    ls, rh = box["ls"], box["rh"]
+    grid = laplace.s_grid(0.01 * TAU, 2e7 * TAU)
+    states = rh.laplace_states({"crust": (30.0, 30.0)}, {"crust": rh.Maxwell(TAU)}, grid[::3])
+    assert states[0].label == "elastic"
+    x, rep = ls.solve(states, [None], rtol=1e-12, n_anchors=3)
+    assert rep.batched.converged.all()
+    stack = x[:, 0]                                      # (n_states, n)
+    pr = laplace.Prony([st.s for st in states[1:]])
+    t = np.concatenate([[0.0], np.logspace(np.log10(0.1 * TAU), np.log10(1e4 * TAU), 7)])
+    W = pr.weights(t)
+    got = laplace.apply_weights(W, stack).cpu().numpy()
+    ref = W @ stack
+    e = float(np.abs(got - ref).max() / np.abs(ref).max())
+    e0 = _rel(got[0], stack[0])
+    print(f"time map: device vs numpy {e:.1e}; u(0+) vs elastic state {e0:.1e}; fit cond "
+          f"{pr.cond:.1e}", flush=True)
+    assert e < 1e-13 and e0 < 1e-9
