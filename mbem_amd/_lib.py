@@ -11,6 +11,7 @@ from __future__ import annotations
 import ctypes
 import os
 
+import numpy as np
 import torch  # noqa: F401  (must precede the library load, see docstring)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
@@ -156,6 +157,13 @@ def require_gpu():
 def device():
     require_gpu()
     return torch.device("cuda", torch.cuda.current_device())
+
+
+def upload(a, dtype):
+    """A host array on the device through a pinned staging copy, without
+    waiting for the work already queued on the stream."""
+    t = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).pin_memory()
+    return t.to(device(), non_blocking=True)
 
 
 def stream():

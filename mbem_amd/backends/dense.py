@@ -213,12 +213,55 @@ class DenseSolveReport:
 
 @dataclass
 class AssemblyReport:
-    """Where the assembly time went (seconds) and how many pairs ran."""
-    seconds: float
+    """Where the assembly time went (seconds) and how many pairs ran.
+    ``seconds`` is the device time from the first assembly launch to the
+    last, read from two HIP events on first access (the assembly returns
+    without waiting for its kernels, so the preconditioner build's launches
+    can queue behind them)."""
+    _events: tuple
     t_pairs_stored: int
     t_pairs_rowsum_only: int
     u_pairs: int
     rhs_pairs: int
+
+    @property
+    def seconds(self) -> float:
+        e0, e1 = self._events
+        e1.synchronize()
+        return e0.elapsed_time(e1) / 1e3
+
+
+class _Timings(dict):
+    """Phase timings; ``assembly_s`` is read from the assembly's events when
+    first looked at."""
+
+    def __init__(self, *a, **k):
+        super().__init__(*a, **k)
+        self._resolve = None
+
+    def _fill(self):
+        if self._resolve is not None and dict.get(self, "assembly_s") is None:
+            dict.__setitem__(self, "assembly_s", self._resolve())
+
+    def __getitem__(self, k):
+        self._fill()
+        return dict.__getitem__(self, k)
+
+    def get(self, k, default=None):
+        self._fill()
+        return dict.get(self, k, default)
+
+    def items(self):
+        self._fill()
+        return dict.items(self)
+
+    def values(self):
+        self._fill()
+        return dict.values(self)
+
+    def __repr__(self):
+        self._fill()
+        return dict.__repr__(self)
 
 
 class AssembledDense:
@@ -234,7 +277,7 @@ class AssembledDense:
         require_anchor_or_deflate(system, jump, deflate)
         warn_half_jump_eps(system, eps, jump)
         warn_collocation_near_fault(system, eps)
-        self.timings = {"checks_s": time.perf_counter() - t0}
+        self.timings = _Timings(checks_s=time.perf_counter() - t0)
         self.system = system
         self.layout = system.layout
         self.eps = eps
@@ -262,8 +305,9 @@ class AssembledDense:
         t0 = time.perf_counter()
         self._prepare_geometry()
         self.timings["geometry_s"] = time.perf_counter() - t0
+        self.timings["assembly_s"] = None
         self._build()
-        self.timings["assembly_s"] = self.assembly.seconds
+        self.timings._resolve = lambda: self.assembly.seconds
 
     # -- geometry on the device ------------------------------------------
     def _prepare_geometry(self):
@@ -321,11 +365,28 @@ class AssembledDense:
         return need
 
     def _build(self):
-        t0 = time.perf_counter()
         torch.cuda.synchronize()
         n = self.layout.n_unknowns
         dev = self.device
         model = self.system.model
+        # every host input goes to the device before the first kernel, so no
+        # copy below waits on the assembly and the host returns at once
+        rhs_in = []
+        for rt in rhs_terms_at(self.system, self.materials):
+            rhs_in.append((rt, torch.as_tensor(np.asarray(rt.vector, float).reshape(-1, 3),
+                                               dtype=F64, device=dev)))
+        rv_in = [(rv, torch.as_tensor(rv.vector, dtype=F64, device=dev))
+                 for rv in self.system.rhs_values]
+        jump_in = []
+        for rt in self.system.rhs_terms:
+            if rt.add_half_of_vector:
+                u_c = np.ascontiguousarray(rt.field_patch.collocation_values(rt.vector).ravel())
+                nq = u_c.size // 3
+                jump_in.append((rt, torch.as_tensor(u_c, dtype=F64, device=dev),
+                                torch.arange(nq + 1, dtype=torch.int64, device=dev)))
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        ev0.record()
         A = torch.empty((n, n), dtype=F64, device=dev)
         lda = A.stride(0)
         n_ts = n_tr = n_u = n_rhs = 0
@@ -399,11 +460,9 @@ class AssembledDense:
             dv.add_block_diag(blk, lda, D)
         # right-hand side (dense.py:514-524)
         b = torch.zeros(n, dtype=F64, device=dev)
-        for rt in rhs_terms_at(self.system, self.materials):
+        for rt, dens in rhs_in:
             x = self._pts[id(rt.field_patch)]
             src = self._src[id(rt.source_patch)]
-            dens = torch.as_tensor(np.asarray(rt.vector, float).reshape(-1, 3), dtype=F64,
-                                   device=dev)
             out = b[rt.row.offset:rt.row.stop].view(-1, 3)
             c = self._coeffs(rt.kernel, rt.region)
             if rt.kernel == KERNEL_T:
@@ -411,18 +470,23 @@ class AssembledDense:
             else:
                 dv.u_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
             n_rhs += x.shape[0] * src.ns
-        for rv in self.system.rhs_values:
-            b[rv.row.offset:rv.row.stop] += torch.as_tensor(rv.vector, dtype=F64, device=dev)
-        if any(rt.add_half_of_vector for rt in self.system.rhs_terms):
-            calib_host = None if self.jump != "calibrated" else {
-                k: v.cpu().numpy() for k, v in calib_dev.items()}
-            bh = np.zeros(n)
-            add_jump_rhs(self.system, calib_host, bh)
-            b += torch.as_tensor(bh, dtype=F64, device=dev)
-        torch.cuda.synchronize()
+        for rv, vec in rv_in:
+            b[rv.row.offset:rv.row.stop] += vec
+        # prescribed-displacement rows: the free term times u_bar moves to the
+        # right-hand side (equations.py:585-603), b -= blockdiag(C_q) u_bar on
+        # the device (a block-CSR product with one diagonal block per row)
+        for rt, u_c, ident in jump_in:
+            r0, r1 = rt.row.offset, rt.row.stop
+            if self.jump != "calibrated":
+                b[r0:r1] += rt.scale_half * u_c
+            else:
+                C = calib_dev[(id(rt.region), id(rt.field_patch))]
+                call("mbem_bsr_gemv", (r1 - r0) // 3, ptr(ident), ptr(ident), ptr(C), ptr(u_c),
+                     ptr(b[r0:r1]), -1.0, stream())
+        ev1.record()
         self.A_dev = A
         self.b_dev = b
-        self.assembly = AssemblyReport(time.perf_counter() - t0, n_ts, n_tr, n_u, n_rhs)
+        self.assembly = AssemblyReport((ev0, ev1), n_ts, n_tr, n_u, n_rhs)
         if self.jump == "calibrated":
             self.calib = _LazyCalib(calib_dev)
 

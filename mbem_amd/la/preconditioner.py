@@ -193,9 +193,11 @@ class BlockGaussSeidel:
         sizes = np.array([c[1].size for c in chunks], dtype=np.int64)
         cptr = np.concatenate([[0], np.cumsum(sizes)])
         iptr = np.concatenate([[0], np.cumsum(sizes * sizes)])
-        self.chunk_ptr = torch.as_tensor(cptr, dtype=I64, device=dev)
-        self.inv_ptr = torch.as_tensor(iptr, dtype=I64, device=dev)
-        self.idx = torch.as_tensor(np.concatenate([c[1] for c in chunks]), dtype=I64, device=dev)
+        # pinned, non-blocking uploads: the host goes on to queue the
+        # inversions behind whatever the stream still runs (the assembly)
+        self.chunk_ptr = _lib.upload(cptr, I64)
+        self.inv_ptr = _lib.upload(iptr, I64)
+        self.idx = _lib.upload(np.concatenate([c[1] for c in chunks]), I64)
         f32 = inverse_dtype == "f32"
         self.inv = torch.empty(int(iptr[-1]), dtype=torch.float32 if f32 else F64, device=dev)
         R = _lib.BJ_ROWS_PER_WAVE
@@ -238,7 +240,7 @@ class BlockGaussSeidel:
             c0 = c1 + 1
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
-            sb.wave_table = torch.as_tensor(wt, dtype=I64, device=dev).contiguous()
+            sb.wave_table = _lib.upload(wt, I64)
             sb.n_waves = wt.shape[0]
         torch.cuda.current_stream().synchronize()
         self.build_s = time.perf_counter() - t0
