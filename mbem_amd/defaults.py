@@ -30,3 +30,6 @@ DENSE_LU = "double"                # defaults.py:1243
 LAPLACE_POINTS_PER_DECADE = 3      # defaults.py:1035
 LAPLACE_PAD_DECADES = 0.5          # defaults.py:1040
 PRONY_RCOND = 1.0e-13              # defaults.py:1067
+LAPLACE_TAIL_DECADES = 12.0        # defaults.py:1062
+EXTENDED_BURGERS_PER_DECADE = 4    # defaults.py:1082
+STEPPER_TAIL_DECADES = 1.0         # defaults.py:1117

@@ -246,10 +246,12 @@ def test_batched_states_match_per_state_lu_and_oracle(box):
         del asm
 
 
-def test_time_map_of_the_batched_stack(box):
+def test_time_map_of_the_batched_stack(box, tmp_path):
     """laplace.py:119-312 on the device stack: the fit's weights applied by
     the library GEMM equal numpy's product to 1e-13, and u(0+) is the elastic
-    state (the fit's first row pins it) to 1e-9."""
+    state (the fit's first row pins it) to 1e-9.  The solved states stored as
+    a run directory (runner.py:318-413 layout) read back by ``run_stack`` /
+    ``run_fit`` (laplace.py:333-367) bit for bit, with the same fit."""
     from mbem_amd import laplace
     ls, rh = box["ls"], box["rh"]
     grid = laplace.s_grid(0.01 * TAU, 2e7 * TAU)
@@ -268,3 +270,12 @@ def test_time_map_of_the_batched_stack(box):
     print(f"time map: device vs numpy {e:.1e}; u(0+) vs elastic state {e0:.1e}; fit cond "
           f"{pr.cond:.1e}", flush=True)
     assert e < 1e-13 and e0 < 1e-9
+    d = laplace.write_run(tmp_path / "maxwell_run", states, [ls.split(v) for v in stack],
+                          {"crust": rh.Maxwell(TAU)},
+                          reports=[{"converged": bool(rep.batched.converged.all())}] * len(states))
+    s_back, fields = laplace.run_stack(d)
+    assert np.array_equal(s_back, np.array([st.s for st in states[1:]]))
+    for i, v in enumerate(stack):
+        for name, arr in ls.split(v).items():
+            assert np.array_equal(fields[name][i], arr)
+    assert np.array_equal(laplace.run_fit(d).coefficients(), pr.coefficients())

@@ -12,8 +12,20 @@ checked against what the mathematics fixes, not against a retyping:
     (checked against a long explicit sum of shifted steps), and the locked
     fault's velocity averages over a cycle to minus the elastic response
     (Savage's back-slip statement), by quadrature;
-  * the guards: secular fits have no relaxed value or memory, tails and
-    repeated points are refused.
+  * the guards: secular fits have no relaxed value or memory, repeated
+    points and malformed tails are refused;
+  * tails (laplace.py:21-49, 86-117): an Andrade body's creep, in closed
+    form J(t) mu = 1 + (t/tau_A)^alpha + t/tau_M, is fitted to ~1e-3 with its
+    tail and ~8x worse without; an extended-Burgers band's creep (the exact
+    sum over its elements) to 2e-5 with the band's tail against 2e-2
+    without; a band the window reaches below is dropped and the fit is the
+    plain one bit for bit; ``memory`` splits the fit into kept poles and the
+    part gone at once, with the elastic instant exact and later times within
+    the bound of what was dropped;
+  * a stored run (laplace.py:315-367): ``write_run`` then ``run_stack`` /
+    ``run_fit`` give back the states in fit order (whatever order they were
+    written in), the tails of the run's rheology, and a MANIFEST of the
+    files' hashes.
 """
 import numpy as np
 import pytest
@@ -100,9 +112,131 @@ def test_guards():
         prs.cycle_weights([1.0], 10.0)
     with pytest.raises(ValueError):
         Prony(s).creep_rate_weights()
-    with pytest.raises(NotImplementedError):
-        Prony(s, tails=(0.3,))
+    for bad in ((1.2,), ((0.3, 1.0, 2.0),), (-0.1,)):
+        with pytest.raises(ValueError):
+            Prony(s, tails=bad)
+    with pytest.raises(ValueError):
+        Prony([1.0, 2.0], tails=(0.2, 0.3))           # no pole left beside two tails
     with pytest.raises(ValueError):
         Prony([1.0, 1.0, 2.0])
     tau, C, gone = Prony(s).memory()
     assert np.array_equal(C, Prony(s).coefficients()) and not gone.any()
+
+
+def _creep_stack(rheo, s):
+    rows = np.concatenate([[np.inf], s])
+    return np.array([1.0 / rheo.shear_ratio(r) for r in rows])    # s J^(s) mu
+
+
+def test_andrade_creep_with_and_without_its_tail():
+    from mbem_amd import rheology as rh
+    s = np.array(s_grid(1.0, 1e3))
+    t = np.logspace(0, 3, 60)
+    for tau_a in (0.3, 1.0, 3.0):
+        a = rh.Andrade(tau_M=300.0, tau_A=tau_a, alpha=1.0 / 3.0)
+        st = _creep_stack(a, s)
+        J = 1.0 + (t / tau_a) ** a.alpha + t / a.tau_M
+        plain = Prony(s, secular=True)
+        tailed = Prony(s, secular=True, tails=rh.tails([a]))
+        assert tailed.tails == (a.alpha,) and tailed.tau.size == plain.tau.size - 1
+        e0 = np.abs(plain.weights(t) @ st - J).max() / J.max()
+        e1 = np.abs(tailed.weights(t) @ st - J).max() / J.max()
+        print(f"Andrade tau_A {tau_a}: creep fit error {e0:.1e} plain, {e1:.1e} with the tail")
+        assert e1 < 2e-3 and e1 < e0 / 5.0
+        # u(0+) is still the elastic state exactly
+        assert abs(tailed.weights(0.0)[0] @ st - st[0]) < 1e-9
+
+
+def test_band_tail_and_a_band_below_the_window():
+    from mbem_amd import rheology as rh
+    s = np.array(s_grid(1.0, 1e3))
+    t = np.logspace(0, 3, 60)
+    for lo, tailed_bar in ((1e-3, 1e-4), (3.0, None)):
+        b = rh.ExtendedBurgers.band(tau_M=1e4, strength=2.0, alpha=0.3, tau_L=lo, tau_H=1e2)
+        st = _creep_stack(b, s)
+        J = (1.0 + np.sum((1.0 - np.exp(-t[:, None] / np.array(b.tau_K)[None, :]))
+                          / np.array(b.kelvin)[None, :], axis=1) + t / b.tau_M)
+        pr = Prony(s, secular=True, tails=(b.tail,))
+        plain = Prony(s, secular=True)
+        e = np.abs(pr.weights(t) @ st - J).max() / J.max()
+        e0 = np.abs(plain.weights(t) @ st - J).max() / J.max()
+        print(f"band from {lo}: tails {pr.tails}, error {e:.1e} (plain {e0:.1e})")
+        if tailed_bar is None:
+            # the window reaches below the band: no comb pole carries any of
+            # it, the tail is dropped and the fit is the plain one
+            assert pr.tails == () and np.array_equal(pr.coefficients(), plain.coefficients())
+            assert e < 1e-3
+        else:
+            assert pr.tails == (b.tail,) and e < tailed_bar and e0 > 100 * e
+
+
+def test_memory_with_a_tail():
+    s = np.array(s_grid(1.0, 1e3))
+    pr = Prony(s, tails=(1.0 / 3.0,))
+    rng = np.random.default_rng(3)
+    st = rng.standard_normal((s.size + 1, 4))
+    tau, C, gone = pr.memory()
+    m = pr.tau.size
+    kept = tau.size - m
+    # one decade of comb poles at three points a decade
+    assert 2 <= kept <= 4 and np.all(1.0 / tau[m:] <= 10.0 * (1 + 1e-9) / pr.tau.min())
+    # the elastic instant: C summed plus gone
+    inst = C.sum(axis=0) + gone
+    assert np.abs(inst @ st - pr.weights(0.0)[0] @ st).max() < 1e-12 * np.abs(st).max() * 1e3
+    # later: the exponentials of memory equal the fit within what was dropped
+    t = np.array([1.0, 3.0, 30.0])
+    mem = np.hstack([np.ones((t.size, 1)), np.exp(-t[:, None] / tau[None, :])]) @ C @ st
+    amp = pr.coefficients()[1 + m] @ st
+    dropped = np.exp(-t[:, None] * pr._rate[None, kept:]) @ pr._w[0, kept:]
+    diff = np.abs(mem - pr.weights(t) @ st)
+    assert np.all(diff <= (dropped[:, None] * np.abs(amp)[None, :]) * (1 + 1e-6) + 1e-12)
+    assert np.all(dropped < 1e-3)
+    # without a tail, memory is the fit itself and nothing is gone
+    tau0, C0, g0 = Prony(s).memory()
+    assert np.array_equal(C0, Prony(s).coefficients()) and not g0.any()
+
+
+def test_run_directory_round_trip(tmp_path):
+    import hashlib
+    import json
+    from mbem_amd import laplace, rheology as rh
+    s = laplace.s_grid(1.0, 1e2)
+    body = rh.Andrade(tau_M=300.0, tau_A=1.0, alpha=1.0 / 3.0)
+    states = rh.laplace_states({"crust": (30.0, 30.0)}, {"crust": body}, s)
+    rng = np.random.default_rng(5)
+    field = rng.standard_normal((7, 3))
+    sols = [{"u:top": field * (1.0 / body.shear_ratio(st.s)), "t:base": field[:2] * k}
+            for k, st in enumerate(states)]
+    order = [3, 0, 5, 1, 2, 4] + list(range(6, len(states)))       # written out of order
+    d = laplace.write_run(tmp_path / "run", [states[i] for i in order], [sols[i] for i in order],
+                          {"crust": body}, reports=[{"iterations": i} for i in order])
+    s_back, stack = laplace.run_stack(d)
+    assert np.array_equal(s_back, np.array(s))
+    assert stack["u:top"].shape == (len(states), 7, 3)
+    for k in range(len(states)):
+        assert np.array_equal(stack["u:top"][k], sols[k]["u:top"])
+        assert np.array_equal(stack["t:base"][k], sols[k]["t:base"])
+    pr = laplace.run_fit(d, secular=True)
+    assert pr.tails == (body.alpha,)
+    ref = Prony(s, secular=True, tails=(body.alpha,))
+    assert np.array_equal(pr.coefficients(), ref.coefficients())
+    resolved = json.loads((d / "resolved.json").read_text())
+    assert resolved["spec"]["laplace"]["rheology"]["crust"]["kind"] == "andrade"
+    assert any(r["s"] == "inf" for r in resolved["states"])
+    for line in (d / "MANIFEST").read_text().splitlines():
+        digest, size, name = line.split()
+        b = (d / name).read_bytes()
+        assert hashlib.sha256(b).hexdigest() == digest and len(b) == int(size)
+    assert {ln.split()[2] for ln in (d / "MANIFEST").read_text().splitlines()} == \
+        {p.name for p in d.iterdir() if p.name != "MANIFEST"}
+    # guards: colliding slugs, no elastic state, not a Laplace run
+    with pytest.raises(ValueError):
+        laplace.write_run(tmp_path / "bad", [states[1], rh.State("s01!", s=1.0)] + [states[0]],
+                          [sols[1], sols[1], sols[0]], {"crust": body})
+    with pytest.raises(ValueError):
+        laplace.write_run(tmp_path / "bad2", states[1:3], sols[1:3], {"crust": body})
+    (tmp_path / "plain").mkdir()
+    (tmp_path / "plain" / "resolved.json").write_text(json.dumps({"states": [{"label": "base"}],
+                                                                  "spec": {}}))
+    with pytest.raises(ValueError):
+        laplace.run_stack(tmp_path / "plain")

@@ -119,3 +119,102 @@ def test_require_one_eps():
     with pytest.raises(ValueError):
         rh.require_one_eps(model, bad, {"crust"})
     rh.require_one_eps(model, bad, set())            # nothing relaxes: no check
+
+
+def test_andrade_modulus_is_the_transform_of_its_creep():
+    """mu / mu(s) = s J^(s) with J(t) mu = 1 + (t/tau_A)^alpha + t/tau_M:
+    the transform by quadrature (rheology.py:77-131), 1e-9; the elastic
+    instant is exact and alpha = 1 is Maxwell at the summed rate."""
+    from scipy import integrate
+    a = rh.Andrade(tau_M=50.0, tau_A=2.0, alpha=1.0 / 3.0)
+    for s in (0.01, 0.3, 7.0):
+        # s * Int exp(-s t) J(t) dt, split at 1/s for the quadrature
+        f = lambda t: math.exp(-s * t) * (1.0 + (t / a.tau_A) ** a.alpha + t / a.tau_M)  # noqa: E731
+        val = s * sum(integrate.quad(f, lo, hi, epsabs=0, epsrel=1e-12, limit=200)[0]
+                      for lo, hi in ((0.0, 1.0 / s), (1.0 / s, 60.0 / s)))
+        assert 1.0 / a.shear_ratio(s) == pytest.approx(val, rel=1e-9)
+    assert a.shear_ratio(math.inf) == 1.0
+    assert a.tail == pytest.approx(1.0 / 3.0) and a.tail_from == 2.0
+    m = rh.Andrade(tau_M=50.0, tau_A=2.0, alpha=1.0)
+    assert m.tail is None and m.tail_from is None
+    for s in (0.01, 1.0):
+        assert m.shear_ratio(s) == pytest.approx(rh.Maxwell(1.0 / (1 / 50.0 + 1 / 2.0)).shear_ratio(s),
+                                                 rel=1e-14)
+    for bad in (dict(tau_M=-1.0, tau_A=1.0, alpha=0.3), dict(tau_M=1.0, tau_A=1.0, alpha=1.5)):
+        with pytest.raises(ValueError):
+            rh.Andrade(**bad)
+
+
+def test_extended_burgers_one_element_and_the_band():
+    """One element is Burgers; ``band`` sums to its strength exactly, its
+    modulus and creep converge on the continuous band (density
+    tau^(alpha - 1) on [tau_L, tau_H], by quadrature in log tau) as the
+    reference's table says (4 nodes a decade: creep 2.5e-5, mu(s) 1.6e-7 of
+    the strength; checked at 1e-4 and 1e-6), and its tail_from is the
+    Andrade tau_A the band's creep follows between its ends."""
+    from scipy import integrate
+    eb = rh.ExtendedBurgers(tau_M=40.0, tau_K=3.0, kelvin=2.5)
+    bg = rh.Burgers(tau_M=40.0, tau_K=3.0, kelvin=2.5)
+    for s in (0.05, 1.0, 20.0):
+        assert eb.shear_ratio(s) == pytest.approx(bg.shear_ratio(s), rel=1e-14)
+    alpha, lo, hi, strength = 0.3, 1e-2, 1e3, 2.0
+    b = rh.ExtendedBurgers.band(tau_M=1e5, strength=strength, alpha=alpha, tau_L=lo, tau_H=hi)
+    assert b.strength == pytest.approx(strength, rel=1e-14)
+    assert len(b.tau_K) == 4 * 5 and b.tail == (alpha, 1.0 / lo, 1.0 / hi) and b.tail_to == lo
+    norm = hi ** alpha - lo ** alpha
+
+    def cont(g):      # strength * Int D(tau) g(tau) dtau over log tau
+        h = lambda u: strength * alpha * math.exp(alpha * u) / norm * g(math.exp(u))  # noqa: E731
+        return integrate.quad(h, math.log(lo), math.log(hi), epsabs=0, epsrel=1e-12, limit=400)[0]
+
+    for s in (1e-3, 0.1, 10.0, 1e3):
+        inv = 1.0 + cont(lambda tau: 1.0 / (1.0 + tau * s)) + 1.0 / (1e5 * s)
+        assert abs(1.0 / b.shear_ratio(s) - inv) < 1e-6 * strength
+    tK, kv = np.array(b.tau_K), np.array(b.kelvin)
+    for t in (0.05, 3.0, 200.0):
+        Jd = np.sum((1.0 - np.exp(-t / tK)) / kv)
+        assert abs(Jd - cont(lambda tau: 1.0 - math.exp(-t / tau))) < 1e-4 * strength
+    # deep inside the band the creep is Andrade's with tau_A = tail_from,
+    # less the part of the power law below tau_L (all relaxed by t >> tau_L:
+    # strength tau_L^alpha / (tau_H^alpha - tau_L^alpha)); the part above
+    # tau_H is (t/tau_H)^(1 - alpha) smaller still
+    wlo, whi = 1e-8, 1e8
+    wide = rh.ExtendedBurgers.band(tau_M=1e5, strength=1.0, alpha=alpha, tau_L=wlo, tau_H=whi,
+                                   per_decade=6)
+    t = 1.0
+    J = np.sum((1.0 - np.exp(-t / np.array(wide.tau_K))) / np.array(wide.kelvin))
+    below = wlo ** alpha / (whi ** alpha - wlo ** alpha)
+    assert J == pytest.approx((t / wide.tail_from) ** alpha - below, rel=1e-4)
+    with pytest.raises(ValueError):
+        rh.ExtendedBurgers(tau_M=1.0, tau_K=(1.0, 2.0), kelvin=(1.0,))
+    with pytest.raises(ValueError):
+        rh.ExtendedBurgers(tau_M=1.0, tau_K=(1.0,), kelvin=(1.0,), power_law=(0.3, 2.0, 5.0))
+
+
+def test_tails_and_dict_forms():
+    """``tails`` (rheology.py:419-431): distinct, sorted, from objects or
+    their dict form; none for bodies with a fastest relaxation."""
+    import dataclasses
+    a = rh.Andrade(tau_M=50.0, tau_A=2.0, alpha=1.0 / 3.0)
+    b = rh.ExtendedBurgers.band(tau_M=1e4, strength=2.0, alpha=0.3, tau_L=1e-3, tau_H=1e2)
+    got = rh.tails([a, dataclasses.asdict(b), {"kind": "maxwell", "tau": 3.0}, a,
+                    rh.Burgers(1.0, 2.0, 3.0)])
+    assert got == ((0.3, 1e3, 1e-2), 1.0 / 3.0)
+    assert rh.from_spec(dataclasses.asdict(b)) == b
+    assert rh.tails([rh.Maxwell(2.0)]) == ()
+
+
+def test_warn_window():
+    """A window whose short end is above tau_A is warned of; a band the
+    window reaches below is not (rheology.py:434-476)."""
+    s = laplace.s_grid(10.0, 1e4)                    # t_min = 10
+    a = rh.Andrade(tau_M=1e5, tau_A=1.0, alpha=1.0 / 3.0)
+    with pytest.warns(UserWarning):
+        said = rh.warn_window({"mantle": a}, s)
+    assert len(said) == 1 and "mantle" in said[0]
+    ok = rh.Andrade(tau_M=1e5, tau_A=20.0, alpha=1.0 / 3.0)
+    band_above = rh.ExtendedBurgers.band(tau_M=1e5, strength=1.0, alpha=0.3, tau_L=20.0, tau_H=1e4)
+    import warnings
+    with warnings.catch_warnings():
+        warnings.simplefilter("error")
+        assert rh.warn_window({"a": ok, "b": band_above}, s) == []
