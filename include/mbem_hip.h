@@ -153,6 +153,13 @@ int mbem_invert(double *M, int64_t m, int32_t *ipiv, int32_t *info, hipStream_t 
 int mbem_invert_batched(double *M, int64_t m, int64_t batch, int32_t *ipiv, int32_t *info,
                         hipStream_t stream);
 
+/* The same inverses written to C (same layout as M, FP32 when f32 != 0,
+ * else FP64): getrf on M in place, then C = A^-1 by solving A C = I with the
+ * factors (rocSOLVER getri_outofplace), which takes far fewer launches than
+ * getri at the chunk sizes of the block-Jacobi rung.  M != C. */
+int mbem_invert_batched_oop(void *M, int64_t m, int64_t batch, int32_t *ipiv, int32_t *info,
+                            void *C, int32_t f32, hipStream_t stream);
+
 /* Padded chunk for one strided-batched inverse of unequal chunks: M (mp x mp,
  * FP64 or FP32) = [A[idx, idx] 0; 0 I]; its inverse's leading m x m block is
  * the chunk's inverse.  And a 2-D strided copy (4- or 8-byte elements) that

@@ -48,6 +48,7 @@ _SIGS = {
     "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert": [_p, _i64, _p, _p, _p],
     "mbem_invert_batched": [_p, _i64, _i64, _p, _p, _p],
+    "mbem_invert_batched_oop": [_p, _i64, _i64, _p, _p, _p, _int, _p],
     "mbem_gather_block_pad": [_p, _i64, _p, _i64, _i64, _p, _int, _p],
     "mbem_copy2d": [_p, _i64, _p, _i64, _i64, _i64, _int, _p],
     "mbem_lu_factor": [_p, _i64, _p, _p, _p],

@@ -557,6 +557,38 @@ int mbem_invert_batched(double* M, int64_t m, int64_t batch, int32_t* ipiv, int3
     return launch_status();
 }
 
+int mbem_invert_batched_oop(void* M, int64_t m, int64_t batch, int32_t* ipiv, int32_t* info,
+                            void* C, int32_t f32, hipStream_t stream)
+{
+    // getrf in place, then the inverse into C by solving A C = I with the
+    // factors (getri_outofplace): a few large triangular solves per batch
+    // instead of getri's long sequence of small blocked steps
+    if (m < 0 || batch < 0 || (m > 0 && batch > 0 && (!M || !ipiv || !info || !C)) ||
+        m * m > INT32_MAX || batch > INT32_MAX || M == C)
+        return MBEM_EINVAL;
+    if (m == 0 || batch == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    rocblas_int mm = (rocblas_int)m, nb = (rocblas_int)batch;
+    rocblas_stride sA = (rocblas_stride)(m * m), sP = (rocblas_stride)m;
+    rocblas_status st;
+    if (f32) {
+        float* A = static_cast<float*>(M);
+        st = rocsolver_sgetrf_strided_batched(g_handle, mm, mm, A, mm, sA, ipiv, sP, info, nb);
+        if (st == rocblas_status_success)
+            st = rocsolver_sgetri_outofplace_strided_batched(g_handle, mm, A, mm, sA, ipiv, sP,
+                                                             static_cast<float*>(C), mm, sA, info, nb);
+    } else {
+        double* A = static_cast<double*>(M);
+        st = rocsolver_dgetrf_strided_batched(g_handle, mm, mm, A, mm, sA, ipiv, sP, info, nb);
+        if (st == rocblas_status_success)
+            st = rocsolver_dgetri_outofplace_strided_batched(g_handle, mm, A, mm, sA, ipiv, sP,
+                                                             static_cast<double*>(C), mm, sA, info, nb);
+    }
+    if (st != rocblas_status_success) return MBEM_EHIP;
+    return launch_status();
+}
+
 // LU factorization of a row-major n x n matrix in place (LAPACK getrf of its
 // column-major view, i.e. of A^T), and the matching solve A x = b (getrs with
 // transpose on that factor).  ipiv: n int32, info: 1 int32, both on device.

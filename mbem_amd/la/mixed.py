@@ -89,8 +89,9 @@ class MixedOperator:
         self.nbr = rowptr.size - 1
         self.nnzb = int(rowptr[-1])
         self.rowptr_h, self.col_h = rowptr, col
-        self.rowptr = torch.as_tensor(rowptr, dtype=I64, device=dev)
-        self.col = torch.as_tensor(col, dtype=I64, device=dev)
+        # pinned, non-blocking uploads: nothing here waits for the device
+        self.rowptr = _lib.upload(rowptr, I64)
+        self.col = _lib.upload(col, I64)
         self.vals = torch.empty(9 * max(self.nnzb, 1), dtype=F64, device=dev)
         align = {"bf16": 8, "fp32": 4, "int8": 16}[storage]
         self.ldf = ((n + align - 1) // align) * align
@@ -117,7 +118,8 @@ class MixedOperator:
             self.far_bytes = n * n * (2 if self.bf16 else 4)
         self._lower = {}
         self.xf = torch.empty(max(n, 4), dtype=torch.float32, device=dev)   # FP32 copy of x
-        torch.cuda.current_stream().synchronize()
+        # host time only: the casts are queued, not waited for (stream order
+        # puts them ahead of any use)
         self.build_s = time.perf_counter() - t0
         self.near_bytes = self.nnzb * (72 + 8)
 
@@ -342,16 +344,36 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     ``recycle`` > 0 keeps up to that many recycled vectors across the inner
     solves (``_RecycledInner``)."""
     from .preconditioner import BlockGaussSeidel
-    # the block-Jacobi build first: its host work (cluster tree, launches)
-    # runs while the assembly still occupies the device, and its gathers are
-    # queued ahead of the operator build, which holds the near blocks out of A
+    # Two streams (both at the default priority).  The near pattern needs
+    # the geometry only: its small kernels run on a side stream beside the
+    # assembly.  The block-Jacobi build's host work (cluster tree, launches)
+    # runs while the assembly occupies the device.  Once its gathers have
+    # read A (``gathered``), the operator build goes on the side stream (near
+    # blocks held out of A, int8 cast, blocks restored), queued before the
+    # inversions' long launch sequence, so its HBM-bound cast runs beside the
+    # inversions, which work on their own copies of the chunks.
     t0 = time.perf_counter()
+    main = torch.cuda.current_stream()
+    side = torch.cuda.Stream()
+    if pattern is None:
+        with torch.cuda.stream(side):
+            pattern = near_structure(assembled, k_near)
+    built = {}
+
+    def start_operator(gathered):
+        side.wait_event(gathered)
+        with torch.cuda.stream(side):
+            built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near,
+                                        pattern=pattern)
+
     M = BlockGaussSeidel(assembled, max_dense=precond_max_dense, bj_chunk=precond_bj_chunk,
-                         grouping=precond_grouping, inverse_dtype="f32")
+                         grouping=precond_grouping, inverse_dtype="f32", defer_check=True,
+                         after_gather=start_operator)
     t1 = time.perf_counter()
-    op = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pattern)
+    op = built["op"]
+    main.wait_stream(side)
     M.coupling = op                      # the lower couplings read the mixed operator
-    torch.cuda.current_stream().synchronize()
+    M.finish()                           # one wait for the assembly and both builds
     t2 = time.perf_counter()
     if recycle:
         inner = _RecycledInner(op, M, inner_rtol, inner_maxiter, op.n, assembled.A_dev.device,

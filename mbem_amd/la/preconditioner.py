@@ -142,7 +142,8 @@ class BlockGaussSeidel:
     def __init__(self, assembled, max_dense=None, bj_chunk=None,
                  grouping: str = defaults.PRECOND_GROUPING,
                  above_dense: str = defaults.PRECOND_RUNG_ABOVE_DENSE, verbose: bool = False,
-                 hodlr_tol=None, hodlr_max=None, inverse_dtype: str = "f64", coupling=None):
+                 hodlr_tol=None, hodlr_max=None, inverse_dtype: str = "f64", coupling=None,
+                 defer_check: bool = False, after_gather=None):
         _lib.require_gpu()
         if max_dense is None:
             max_dense = dense_rung_max_dof()
@@ -207,46 +208,85 @@ class BlockGaussSeidel:
         # invert the chunks in size classes (largest / smallest <= 1.1): each
         # class is padded to its largest size with an identity block and
         # inverted in one strided-batched getrf + getri, then the leading
-        # blocks are packed into ``inv``
+        # blocks are packed into ``inv``.  Every class is gathered first, and
+        # ``gathered`` (an event on the stream) marks the last read of A, so a
+        # caller may change A on another stream after it while the inversions
+        # run on the padded copies.
         order = np.argsort(sizes, kind="stable")
         esz = 4 if f32 else 8
+        classes = []
         c0 = 0
         while c0 < len(order):
             c1 = c0
             while c1 + 1 < len(order) and sizes[order[c1 + 1]] <= 1.1 * sizes[order[c0]]:
                 c1 += 1
             cls = order[c0:c1 + 1]
-            mp, nb = int(sizes[cls].max()), len(cls)
-            tmp = torch.empty((nb, mp, mp), dtype=torch.float32 if f32 else F64, device=dev)
+            mp = int(sizes[cls].max())
+            tmp = torch.empty((len(cls), mp, mp), dtype=torch.float32 if f32 else F64, device=dev)
             for q, ci in enumerate(cls):
                 call("mbem_gather_block_pad", ptr(A), A.stride(0),
                      ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), int(sizes[ci]), mp,
                      ptr(tmp[q]), int(f32), stream())
+            classes.append((cls, mp, tmp))
+            c0 = c1 + 1
+        self.gathered = torch.cuda.Event()
+        self.gathered.record()
+        if after_gather is not None:
+            # the caller's own launches (e.g. on another stream, after
+            # ``gathered``) go ahead of the inversions' long launch sequence
+            after_gather(self.gathered)
+        # getrf's info flags, read back once after every class is queued
+        # (``finish``), so no class waits on the one before.  The inverse of
+        # each class goes to a second buffer (getrf, then A C = I solved with
+        # the factors), then its leading blocks into ``inv``.
+        self._pending = []
+        for cls, mp, tmp in classes:
+            nb = len(cls)
             ipiv = torch.empty(nb * mp, dtype=torch.int32, device=dev)
             info = torch.zeros(nb, dtype=torch.int32, device=dev)
-            call("mbem_invert_batched_f32" if f32 else "mbem_invert_batched", ptr(tmp), mp, nb,
-                 ptr(ipiv), ptr(info), stream())
-            bad = info.cpu().numpy()
-            if np.any(bad != 0):
-                q = int(np.flatnonzero(bad)[0])
-                raise np.linalg.LinAlgError(f"block-Jacobi chunk {int(cls[q])} "
-                                            f"({int(sizes[cls[q]])} DOFs) is singular "
-                                            f"(getrf info {int(bad[q])})")
+            out = torch.empty_like(tmp)
+            call("mbem_invert_batched_oop", ptr(tmp), mp, nb, ptr(ipiv), ptr(info), ptr(out),
+                 int(f32), stream())
+            self._pending.append((info, cls, sizes[cls]))
             for q, ci in enumerate(cls):
                 m = int(sizes[ci])
-                call("mbem_copy2d", ptr(tmp[q]), mp, ptr(self.inv[int(iptr[ci]):]), m, m, m, esz,
+                call("mbem_copy2d", ptr(out[q]), mp, ptr(self.inv[int(iptr[ci]):]), m, m, m, esz,
                      stream())
-            del tmp
-            c0 = c1 + 1
+        del classes
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
             sb.wave_table = _lib.upload(wt, I64)
             sb.n_waves = wt.shape[0]
-        torch.cuda.current_stream().synchronize()
-        self.build_s = time.perf_counter() - t0
+        self._t0 = t0
+        self.build_s = None
+        if not defer_check:
+            self.finish()
         self.n_chunks = len(chunks)
         self.inverse_bytes = int(iptr[-1]) * (4 if f32 else 8)
         self._rr = torch.empty(n, dtype=F64, device=dev)
+
+    def finish(self) -> "BlockGaussSeidel":
+        """Wait for the chunk inversions and raise ``LinAlgError`` for the first
+        singular chunk (getrf's info), as the build would have at once.  Called
+        by the constructor unless ``defer_check``; a caller that defers it
+        queues more device work first and calls this before the first apply.
+        ``build_s`` is then the time from the build's start to here."""
+        if self._pending:
+            flags = torch.cat([p[0] for p in self._pending]).cpu().numpy()
+            o = 0
+            for info, cls, sz in self._pending:
+                bad = flags[o:o + cls.size]
+                o += cls.size
+                if np.any(bad != 0):
+                    q = int(np.flatnonzero(bad)[0])
+                    raise np.linalg.LinAlgError(f"block-Jacobi chunk {int(cls[q])} "
+                                                f"({int(sz[q])} DOFs) is singular "
+                                                f"(getrf info {int(bad[q])})")
+            self._pending = []
+        torch.cuda.current_stream().synchronize()
+        if self.build_s is None:
+            self.build_s = time.perf_counter() - self._t0
+        return self
 
     def apply_into(self, r: torch.Tensor, z: torch.Tensor) -> torch.Tensor:
         """z = M(r): one forward sweep (preconditioner.py:521-543)."""
