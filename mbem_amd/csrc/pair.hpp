@@ -46,22 +46,49 @@ __device__ __forceinline__ void load_src(const double* __restrict__ src, long ld
     S.ok = src[F_OK * ld + s] != 0.0;
 }
 
-// ---- edge antiderivatives: tri_kernels.py:54-76, :606-619 ----------------
-__device__ __forceinline__ double fJ1(double u, double rho2) { return log(u + sqrt(u * u + rho2) + 1e-300); }
-__device__ __forceinline__ double fJ3(double u, double rho2, double R)
+// ---- reciprocal and square root -------------------------------------------
+// The pair integrals divide by, and take roots of, positive lengths, heights
+// and their powers, far from the denormal and overflow ranges, so the IEEE
+// special-case paths of '/' and sqrt (div_scale / div_fmas / div_fixup, the
+// range scaling of sqrt) are left out: the v_rcp_f64 / v_rsq_f64 seeds are
+// refined by Newton steps (each squares the relative error), which leaves
+// about one ulp.
+__device__ __forceinline__ double frcp(double x)
 {
-    return (rho2 < 1e-60) ? 0.0 : u / (rho2 * R);
+    double r = __builtin_amdgcn_rcp(x);
+    double e = fma(-x, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-x, r, 1.0);
+    return fma(r, e, r);
 }
-__device__ __forceinline__ double fK3(double R) { return -1.0 / (R < 1e-30 ? 1e-30 : R); }
-__device__ __forceinline__ double fJ5(double u, double rho2, double R)
+// sq = sqrt(x) and is = 1/sqrt(x), x > 0: one Goldschmidt step on g = x y,
+// h = y / 2 from the seed y, then one correction of each
+__device__ __forceinline__ void frsq(double x, double& sq, double& is)
 {
-    return (rho2 < 1e-60) ? 0.0 : u * (3.0 * rho2 + 2.0 * u * u) / (3.0 * rho2 * rho2 * R * R * R);
+    double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double r = fma(-g, h, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    r = fma(-g, h, 0.5);
+    h = fma(h, r, h);
+    sq = g;
+    is = 2.0 * h;
 }
-__device__ __forceinline__ double fK5(double R) { return (R < 1e-30) ? 0.0 : -1.0 / (3.0 * R * R * R); }
+__device__ __forceinline__ double fsqrt(double x)
+{
+    double sq, is;
+    frsq(x, sq, is);
+    return sq;
+}
+constexpr double kThird = 1.0 / 3.0;
+constexpr double kFifth = 0.2;
 
 // ---- per-pair geometry: tri_kernels.py:163-182 ------------------------------
 struct Geom {
-    double z, h2, I3;
+    double z, h2, ih2, I3;
     double px[3], py[3];
 };
 
@@ -71,9 +98,11 @@ __device__ __forceinline__ double solid_angle(const double* v, double ox, double
     double r1x = v[0] - ox, r1y = v[1] - oy, r1z = v[2] - oz;
     double r2x = v[3] - ox, r2y = v[4] - oy, r2z = v[5] - oz;
     double r3x = v[6] - ox, r3y = v[7] - oy, r3z = v[8] - oz;
-    double R1 = sqrt(r1x * r1x + r1y * r1y + r1z * r1z);
-    double R2 = sqrt(r2x * r2x + r2y * r2y + r2z * r2z);
-    double R3 = sqrt(r3x * r3x + r3y * r3y + r3z * r3z);
+    // the lifted point is h >= eps > 0 off the plane: every R > 0 (the floor
+    // only keeps a zero-width call finite)
+    double R1 = fsqrt(fmax(r1x * r1x + r1y * r1y + r1z * r1z, 1e-300));
+    double R2 = fsqrt(fmax(r2x * r2x + r2y * r2y + r2z * r2z, 1e-300));
+    double R3 = fsqrt(fmax(r3x * r3x + r3y * r3y + r3z * r3z, 1e-300));
     double cx = r2y * r3z - r2z * r3y;
     double cy = r2z * r3x - r2x * r3z;
     double cz = r2x * r3y - r2y * r3x;
@@ -90,7 +119,13 @@ __device__ __forceinline__ void geom(const Src& S, double ox, double oy, double 
     double dx = ox - S.v[0], dy = oy - S.v[1], dz = oz - S.v[2];
     double z = dx * S.nh[0] + dy * S.nh[1] + dz * S.nh[2];
     double h2 = z * z + S.eps * S.eps;
-    double h = sqrt(h2);
+    double h, ih;
+    if (h2 > 0.0) {
+        frsq(h2, h, ih);
+    } else {
+        h = 0.0;
+        ih = 1.0 / h2;
+    }
     double opx = ox - z * S.nh[0];
     double opy = oy - z * S.nh[1];
     double opz = oz - z * S.nh[2];
@@ -101,9 +136,10 @@ __device__ __forceinline__ void geom(const Src& S, double ox, double oy, double 
         g.py[k] = wx * S.ey[0] + wy * S.ey[1] + wz * S.ey[2];
     }
     double Omega = solid_angle(S.v, opx + h * S.nh[0], opy + h * S.nh[1], opz + h * S.nh[2]);
-    g.I3 = -Omega / h;
+    g.I3 = -Omega * ih;
     g.z = z;
     g.h2 = h2;
+    g.ih2 = ih * ih;
 }
 
 // One edge in the edge-local frame (tri_kernels.py:190-201): outward normal
@@ -118,17 +154,50 @@ __device__ __forceinline__ void edge(const Geom& g, int e, Edge& E)
     int e1 = (e == 2) ? 0 : e + 1;
     double ax = g.px[e], ay = g.py[e], bx = g.px[e1], by = g.py[e1];
     double evx = bx - ax, evy = by - ay;
-    double L = sqrt(evx * evx + evy * evy);
-    E.live = L > 1e-30;
-    double Ls = E.live ? L : 1.0;
-    E.tx = evx / Ls;
-    E.ty = evy / Ls;
+    double L2 = evx * evx + evy * evy;
+    E.live = L2 > 1e-60;                 // L > 1e-30
+    double L, iL;
+    frsq(E.live ? L2 : 1.0, L, iL);
+    E.tx = E.live ? evx * iL : evx;
+    E.ty = E.live ? evy * iL : evy;
     E.nx = E.ty;
     E.ny = -E.tx;
     E.dp = ax * E.nx + ay * E.ny;
     E.ua = ax * E.tx + ay * E.ty;
     E.ub = bx * E.tx + by * E.ty;
     E.rho2 = E.dp * E.dp + g.h2;
+}
+
+// ---- edge antiderivatives: tri_kernels.py:54-76, :606-619 ----------------
+// One edge's endpoint terms (tri_kernels.py:54-64, same guards):
+// R = sqrt(u^2 + rho2) and 1/R at both ends; J1(b) - J1(a) as one log of the
+// ratio of the two arguments; J3(b) - J3(a) = (ub/Rb - ua/Ra) / rho2 (0 when
+// rho2 < 1e-60); K3(b) - K3(a) with K3 = -1/max(R, 1e-30).
+struct Ends {
+    double Ra, Rb, iRa, iRb, irho2, dJ1, dJ3, dK3;
+};
+
+__device__ __forceinline__ void ends(const Edge& E, Ends& P)
+{
+    frsq(fmax(E.ua * E.ua + E.rho2, 1e-300), P.Ra, P.iRa);
+    frsq(fmax(E.ub * E.ub + E.rho2, 1e-300), P.Rb, P.iRb);
+    P.dJ1 = log((E.ub + P.Rb + 1e-300) * frcp(E.ua + P.Ra + 1e-300));
+    P.irho2 = (E.rho2 < 1e-60) ? 0.0 : frcp(E.rho2);
+    P.dJ3 = (E.ub * P.iRb - E.ua * P.iRa) * P.irho2;
+    P.dK3 = fmin(P.iRa, 1e30) - fmin(P.iRb, 1e30);
+}
+
+// J5(b) - J5(a), J5 = u (3 rho2 + 2 u^2) / (3 rho2^2 R^3), and K5(b) - K5(a),
+// K5 = -1 / (3 R^3) (0 for R < 1e-30): tri_kernels.py:606-619
+__device__ __forceinline__ void ends5(const Edge& E, const Ends& P, double& dJ5, double& dK5)
+{
+    double ia3 = P.iRa * P.iRa * P.iRa, ib3 = P.iRb * P.iRb * P.iRb;
+    double c = kThird * P.irho2 * P.irho2;
+    dJ5 = c * (E.ub * (3.0 * E.rho2 + 2.0 * E.ub * E.ub) * ib3
+               - E.ua * (3.0 * E.rho2 + 2.0 * E.ua * E.ua) * ia3);
+    double ka = (P.iRa > 1e30) ? 0.0 : kThird * ia3;
+    double kb = (P.iRb > 1e30) ? 0.0 : kThird * ib3;
+    dK5 = ka - kb;
 }
 
 // Edge integrals of xi1^a xi2^b / R^m (tri_kernels.py:99-129, :632-652)
@@ -192,9 +261,10 @@ __device__ __forceinline__ void u_moments(const Src& S, double ox, double oy, do
         Edge E;
         edge(g, e, E);
         if (!E.live) continue;
-        double Ra = sqrt(E.ua * E.ua + E.rho2), Rb = sqrt(E.ub * E.ub + E.rho2);
-        double dJ1 = log(E.ub + Rb + 1e-300) - log(E.ua + Ra + 1e-300);
-        double dK1 = Rb - Ra;
+        Ends P;
+        ends(E, P);
+        double dJ1 = P.dJ1;
+        double dK1 = P.Rb - P.Ra;
         EI ei{E.dp, E.nx, E.ny, E.tx, E.ty};
         double v00 = ei.e00(dJ1);
         double v10 = ei.e10(dJ1, dK1);
@@ -275,11 +345,9 @@ __device__ __forceinline__ void t_moments(const Src& S, double ox, double oy, do
         Edge E;
         edge(g, e, E);
         if (!E.live) continue;
-        double Ra = sqrt(E.ua * E.ua + E.rho2), Rb = sqrt(E.ub * E.ub + E.rho2);
-        double dJ1 = log(E.ub + Rb + 1e-300) - log(E.ua + Ra + 1e-300);
-        double dK1 = Rb - Ra;
-        double dJ3 = fJ3(E.ub, E.rho2, Rb) - fJ3(E.ua, E.rho2, Ra);
-        double dK3 = fK3(Rb) - fK3(Ra);
+        Ends P;
+        ends(E, P);
+        double dJ1 = P.dJ1, dJ3 = P.dJ3, dK3 = P.dK3;
         double dL3 = dJ1 - E.rho2 * dJ3;
         EI ei{E.dp, E.nx, E.ny, E.tx, E.ty};
         double v00m1 = dJ1;
@@ -299,16 +367,16 @@ __device__ __forceinline__ void t_moments(const Src& S, double ox, double oy, do
         BN1_023 += E.nx * v02; BN2_023 += E.ny * v02;
     }
     double I3 = g.I3, z = g.z;
-    double I5 = (BD003 + I3) / (3.0 * g.h2);
+    double I5 = (BD003 + I3) * kThird * g.ih2;
     double m10_3 = -BN1_001, m01_3 = -BN2_001;
-    double m10 = -BN1_003 / 3.0, m01 = -BN2_003 / 3.0;
-    double m20 = (I3 - BN1_103) / 3.0;
-    double m11 = -BN1_013 / 3.0;
-    double m02 = (I3 - BN2_013) / 3.0;
-    double m30 = (2.0 * m10_3 - BN1_203) / 3.0;
-    double m21 = (m01_3 - BN1_113) / 3.0;
-    double m12 = -BN1_023 / 3.0;
-    double m03 = (2.0 * m01_3 - BN2_023) / 3.0;
+    double m10 = -BN1_003 * kThird, m01 = -BN2_003 * kThird;
+    double m20 = (I3 - BN1_103) * kThird;
+    double m11 = -BN1_013 * kThird;
+    double m02 = (I3 - BN2_013) * kThird;
+    double m30 = (2.0 * m10_3 - BN1_203) * kThird;
+    double m21 = (m01_3 - BN1_113) * kThird;
+    double m12 = -BN1_023 * kThird;
+    double m03 = (2.0 * m01_3 - BN2_023) * kThird;
     // local d = (-xi1, -xi2, z)
     T.V3[0] = -m10_3; T.V3[1] = -m01_3; T.V3[2] = z * I3;
     T.V5[0] = -m10;   T.V5[1] = -m01;   T.V5[2] = z * I5;
@@ -415,10 +483,9 @@ __device__ __forceinline__ void s_moments(const Src& S, double ox, double oy, do
         Edge E;
         edge(g, e, E);
         if (!E.live) continue;
-        double Ra = sqrt(E.ua * E.ua + E.rho2), Rb = sqrt(E.ub * E.ub + E.rho2);
-        double dJ1 = log(E.ub + Rb + 1e-300) - log(E.ua + Ra + 1e-300);
-        double dJ3 = fJ3(E.ub, E.rho2, Rb) - fJ3(E.ua, E.rho2, Ra);
-        double dK3 = fK3(Rb) - fK3(Ra);
+        Ends P;
+        ends(E, P);
+        double dJ1 = P.dJ1, dJ3 = P.dJ3, dK3 = P.dK3;
         double dL3 = dJ1 - E.rho2 * dJ3;
         EI ei{E.dp, E.nx, E.ny, E.tx, E.ty};
         BN1m1 += E.nx * dJ1; BN2m1 += E.ny * dJ1;
@@ -430,8 +497,8 @@ __device__ __forceinline__ void s_moments(const Src& S, double ox, double oy, do
         v = ei.e11(dJ3, dK3, dL3); b1_11 += E.nx * v;
         v = ei.e20(dJ3, dK3, dL3); b1_20 += E.nx * v;
         if (NEED7) {
-            double dJ5 = fJ5(E.ub, E.rho2, Rb) - fJ5(E.ua, E.rho2, Ra);
-            double dK5 = fK5(Rb) - fK5(Ra);
+            double dJ5, dK5;
+            ends5(E, P, dJ5, dK5);
             double dL5 = dJ3 - E.rho2 * dJ5;
             double dM5 = dK3 - E.rho2 * dK5;
             v = dJ5; c1_00 += E.nx * v; c2_00 += E.ny * v; BD005 += E.dp * v;
@@ -447,7 +514,7 @@ __device__ __forceinline__ void s_moments(const Src& S, double ox, double oy, do
         }
     }
     double h2 = g.h2, I3 = g.I3;
-    double I5 = (h2 > 1e-60) ? (BD003 + I3) / (3.0 * h2) : 0.0;
+    double I5 = (h2 > 1e-60) ? (BD003 + I3) * kThird * g.ih2 : 0.0;
     M.z = g.z;
     M.I3 = I3;
     M.I5 = I5;
@@ -455,35 +522,35 @@ __device__ __forceinline__ void s_moments(const Src& S, double ox, double oy, do
     M.m3_01 = -BN2m1;
     double* m5 = M.m5;
     m5[0] = I5;
-    m5[1] = -b1_00 / 3.0;
-    m5[2] = -b2_00 / 3.0;
-    m5[3] = (I3 - b1_10) / 3.0;
-    m5[4] = -b1_01 / 3.0;
-    m5[5] = (I3 - b2_01) / 3.0;
-    m5[6] = (2.0 * M.m3_10 - b1_20) / 3.0;
-    m5[7] = (M.m3_01 - b1_11) / 3.0;
-    m5[8] = -b1_02 / 3.0;
-    m5[9] = (2.0 * M.m3_01 - b2_02) / 3.0;
+    m5[1] = -b1_00 * kThird;
+    m5[2] = -b2_00 * kThird;
+    m5[3] = (I3 - b1_10) * kThird;
+    m5[4] = -b1_01 * kThird;
+    m5[5] = (I3 - b2_01) * kThird;
+    m5[6] = (2.0 * M.m3_10 - b1_20) * kThird;
+    m5[7] = (M.m3_01 - b1_11) * kThird;
+    m5[8] = -b1_02 * kThird;
+    m5[9] = (2.0 * M.m3_01 - b2_02) * kThird;
     M.I7 = 0.0;
     if (NEED7) {
-        double I7 = (h2 > 1e-60) ? (BD005 + 3.0 * I5) / (5.0 * h2) : 0.0;
+        double I7 = (h2 > 1e-60) ? (BD005 + 3.0 * I5) * kFifth * g.ih2 : 0.0;
         M.I7 = I7;
         double* m7 = M.m7;
         m7[0] = I7;
-        m7[1] = -c1_00 / 5.0;
-        m7[2] = -c2_00 / 5.0;
-        m7[3] = (I5 - c1_10) / 5.0;
-        m7[4] = -c1_01 / 5.0;
-        m7[5] = (I5 - c2_01) / 5.0;
-        m7[6] = (2.0 * m5[1] - c1_20) / 5.0;
-        m7[7] = (m5[2] - c1_11) / 5.0;
-        m7[8] = -c1_02 / 5.0;
-        m7[9] = (2.0 * m5[2] - c2_02) / 5.0;
-        m7[10] = (3.0 * m5[3] - c1_30) / 5.0;
-        m7[11] = (2.0 * m5[4] - c1_21) / 5.0;
-        m7[12] = (m5[5] - c1_12) / 5.0;
-        m7[13] = -c1_03 / 5.0;
-        m7[14] = (3.0 * m5[5] - c2_03) / 5.0;
+        m7[1] = -c1_00 * kFifth;
+        m7[2] = -c2_00 * kFifth;
+        m7[3] = (I5 - c1_10) * kFifth;
+        m7[4] = -c1_01 * kFifth;
+        m7[5] = (I5 - c2_01) * kFifth;
+        m7[6] = (2.0 * m5[1] - c1_20) * kFifth;
+        m7[7] = (m5[2] - c1_11) * kFifth;
+        m7[8] = -c1_02 * kFifth;
+        m7[9] = (2.0 * m5[2] - c2_02) * kFifth;
+        m7[10] = (3.0 * m5[3] - c1_30) * kFifth;
+        m7[11] = (2.0 * m5[4] - c1_21) * kFifth;
+        m7[12] = (m5[5] - c1_12) * kFifth;
+        m7[13] = -c1_03 * kFifth;
+        m7[14] = (3.0 * m5[5] - c2_03) * kFifth;
     }
 }
 
