@@ -210,7 +210,82 @@ __global__ __launch_bounds__(256) void k_basis(const double* __restrict__ xf, lo
 // ---- matrix-free contractions: tri_kernels.py:976-1049, :1068-1154, :1217-1260
 // KIND 0: U displacement, 1: T displacement, 2: Kelvin stress, 3: dislocation
 // (total) stress, 4: eigenstress.  out is (nf, 3) or (nf, 3, 3); out = scale *
-// result + beta * out.
+// result + beta * out.  Zero-density, degenerate and (eigenstress) eps <= 0
+// sources are skipped.
+
+// one source's contribution at one point: acc += K(x, source) d
+template <int KIND>
+__device__ __forceinline__ void contract_pair(const Src& S, const double* d, double ox, double oy,
+                                              double oz, const Coef& cf, double mu, double nu,
+                                              double lam, double* acc)
+{
+    if (KIND == 0) {
+        UMom M;
+        u_moments(S, ox, oy, oz, M);
+        double u[9];
+        u_direct(S, M, cf.c[0], cf.c[1], cf.c[2], u);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += u[i * 3 + 0] * d[0] + u[i * 3 + 1] * d[1] + u[i * 3 + 2] * d[2];
+    } else if (KIND == 1) {
+        TMom T;
+        t_moments(S, ox, oy, oz, T);
+        double L[9], dl[3], ul[3];
+        t_combine_local(T, S.eps, cf.c, L);
+        to_local(S, d, dl);
+#pragma unroll
+        for (int i = 0; i < 3; ++i) ul[i] = L[i * 3 + 0] * dl[0] + L[i * 3 + 1] * dl[1] + L[i * 3 + 2] * dl[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) acc[i] += S.ex[i] * ul[0] + S.ey[i] * ul[1] + S.nh[i] * ul[2];
+    } else if (KIND == 2) {
+        SMom M;
+        s_moments<false>(S, ox, oy, oz, M);
+        double fl[3], sl[6], g[9];
+        to_local(S, d, fl);
+        kelvin_local(M, S.eps, mu, nu, lam, fl, sl);
+        rot_sym(S, sl, g);
+        acc[0] += g[0]; acc[1] += g[1]; acc[2] += g[2]; acc[3] += g[4]; acc[4] += g[5]; acc[5] += g[8];
+    } else if (KIND == 3) {
+        SMom M;
+        s_moments<true>(S, ox, oy, oz, M);
+        double bl[3], nl[3], sl[6], g[9];
+        to_local(S, d, bl);
+        to_local(S, S.n, nl);
+        dd_local(M, S.eps, mu, nu, lam, nl, bl, sl);
+        rot_sym(S, sl, g);
+        acc[0] += g[0]; acc[1] += g[1]; acc[2] += g[2]; acc[3] += g[4]; acc[4] += g[5]; acc[5] += g[8];
+    } else {
+        SMom M;
+        s_moments<true>(S, ox, oy, oz, M);
+        double sg[6];
+        eigen_global(M.I7, S.eps, mu, lam, S.n, d, sg);
+#pragma unroll
+        for (int q = 0; q < 6; ++q) acc[q] += sg[q];
+    }
+}
+
+template <int NO>
+__device__ __forceinline__ void contract_store(const double* acc, long f, double* out, double scale,
+                                               double beta)
+{
+    if (NO == 3) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            double* p = out + 3 * f + i;
+            *p = (beta == 0.0) ? scale * acc[i] : scale * acc[i] + beta * (*p);
+        }
+    } else {
+        // packed 00 01 02 11 12 22 -> full 3x3
+        const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
+#pragma unroll
+        for (int q = 0; q < 9; ++q) {
+            double* p = out + 9 * f + q;
+            *p = (beta == 0.0) ? scale * acc[map[q]] : scale * acc[map[q]] + beta * (*p);
+        }
+    }
+}
+
+// lane = point, the source loop wave-uniform (scalar loads of each source
+// record): for many points
 template <int KIND>
 __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ xf, long nf,
                                                   const double* __restrict__ src, long lds, long ns,
@@ -228,72 +303,50 @@ __global__ __launch_bounds__(256) void k_contract(const double* __restrict__ xf,
 #pragma unroll
     for (int q = 0; q < NO; ++q) acc[q] = 0.0;
     for (long s = 0; s < ns; ++s) {
-        const double d0 = dens[3 * s], d1 = dens[3 * s + 1], d2 = dens[3 * s + 2];
-        if (d0 == 0.0 && d1 == 0.0 && d2 == 0.0) continue;
+        const double d[3] = {dens[3 * s], dens[3 * s + 1], dens[3 * s + 2]};
+        if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
         Src S;
         load_src(src, lds, s, S);
         if (!S.ok) continue;
         if (KIND == 4 && !(S.eps > 0.0)) continue;
-        double d[3] = {d0, d1, d2};
-        if (KIND == 0) {
-            UMom M;
-            u_moments(S, ox, oy, oz, M);
-            double u[9];
-            u_direct(S, M, cf.c[0], cf.c[1], cf.c[2], u);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) acc[i] += u[i * 3 + 0] * d0 + u[i * 3 + 1] * d1 + u[i * 3 + 2] * d2;
-        } else if (KIND == 1) {
-            TMom T;
-            t_moments(S, ox, oy, oz, T);
-            double L[9], dl[3], ul[3];
-            t_combine_local(T, S.eps, cf.c, L);
-            to_local(S, d, dl);
-#pragma unroll
-            for (int i = 0; i < 3; ++i) ul[i] = L[i * 3 + 0] * dl[0] + L[i * 3 + 1] * dl[1] + L[i * 3 + 2] * dl[2];
-#pragma unroll
-            for (int i = 0; i < 3; ++i) acc[i] += S.ex[i] * ul[0] + S.ey[i] * ul[1] + S.nh[i] * ul[2];
-        } else if (KIND == 2) {
-            SMom M;
-            s_moments<false>(S, ox, oy, oz, M);
-            double fl[3], sl[6], g[9];
-            to_local(S, d, fl);
-            kelvin_local(M, S.eps, mu, nu, lam, fl, sl);
-            rot_sym(S, sl, g);
-            acc[0] += g[0]; acc[1] += g[1]; acc[2] += g[2]; acc[3] += g[4]; acc[4] += g[5]; acc[5] += g[8];
-        } else if (KIND == 3) {
-            SMom M;
-            s_moments<true>(S, ox, oy, oz, M);
-            double bl[3], nl[3], sl[6], g[9];
-            to_local(S, d, bl);
-            to_local(S, S.n, nl);
-            dd_local(M, S.eps, mu, nu, lam, nl, bl, sl);
-            rot_sym(S, sl, g);
-            acc[0] += g[0]; acc[1] += g[1]; acc[2] += g[2]; acc[3] += g[4]; acc[4] += g[5]; acc[5] += g[8];
-        } else {
-            SMom M;
-            s_moments<true>(S, ox, oy, oz, M);
-            double sg[6];
-            eigen_global(M.I7, S.eps, mu, lam, S.n, d, sg);
-#pragma unroll
-            for (int q = 0; q < 6; ++q) acc[q] += sg[q];
-        }
+        contract_pair<KIND>(S, d, ox, oy, oz, cf, mu, nu, lam, acc);
     }
     if (!live) return;
-    if (NO == 3) {
+    contract_store<NO>(acc, f, out, scale, beta);
+}
+
+// one wave per point, lane = source, a wave reduction at the end: for few
+// points (a right-hand side over one patch's collocation points, a grid
+// readout), where a lane per point leaves most of the device idle
+template <int KIND>
+__global__ __launch_bounds__(256) void k_contract_rows(const double* __restrict__ xf, long nf,
+                                                       const double* __restrict__ src, long lds,
+                                                       long ns, const double* __restrict__ dens,
+                                                       Coef cf, double mu, double lam,
+                                                       double* __restrict__ out, double scale,
+                                                       double beta)
+{
+    const int lane = threadIdx.x & 63;
+    const long f = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (f >= nf) return;
+    const double ox = xf[3 * f], oy = xf[3 * f + 1], oz = xf[3 * f + 2];
+    const double nu = lam / (2.0 * (lam + mu));
+    constexpr int NO = (KIND <= 1) ? 3 : 6;
+    double acc[NO];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) {
-            double* p = out + 3 * f + i;
-            *p = (beta == 0.0) ? scale * acc[i] : scale * acc[i] + beta * (*p);
-        }
-    } else {
-        // packed 00 01 02 11 12 22 -> full 3x3
-        const int map[9] = {0, 1, 2, 1, 3, 4, 2, 4, 5};
-#pragma unroll
-        for (int q = 0; q < 9; ++q) {
-            double* p = out + 9 * f + q;
-            *p = (beta == 0.0) ? scale * acc[map[q]] : scale * acc[map[q]] + beta * (*p);
-        }
+    for (int q = 0; q < NO; ++q) acc[q] = 0.0;
+    for (long s = lane; s < ns; s += 64) {
+        const double d[3] = {dens[3 * s], dens[3 * s + 1], dens[3 * s + 2]};
+        if (d[0] == 0.0 && d[1] == 0.0 && d[2] == 0.0) continue;
+        Src S;
+        load_src(src, lds, s, S);
+        if (!S.ok) continue;
+        if (KIND == 4 && !(S.eps > 0.0)) continue;
+        contract_pair<KIND>(S, d, ox, oy, oz, cf, mu, nu, lam, acc);
     }
+#pragma unroll
+    for (int q = 0; q < NO; ++q) acc[q] = wave_sum(acc[q]);
+    if (lane == 0) contract_store<NO>(acc, f, out, scale, beta);
 }
 
 // eigenstress on a near list (evaluate.py:298-326): point p sums the source
@@ -376,14 +429,24 @@ int launch_block(const double* xf, long nf, const double* src, long lds, long ns
     return launch_status();
 }
 
+constexpr long kRowsBelow = 64 * 1024;
+
 template <int KIND>
 int launch_contract(const double* xf, long nf, const double* src, long lds, long ns,
                     const double* dens, const Coef& cf, double mu, double lam, double* out,
                     double scale, double beta, hipStream_t st)
 {
     if (nf == 0) return MBEM_OK;
-    dim3 grid((unsigned)((nf + 255) / 256)), block(256);
-    hipLaunchKernelGGL((k_contract<KIND>), grid, block, 0, st, xf, nf, src, lds, ns, dens, cf, mu, lam, out, scale, beta);
+    // a lane per point gives nf / 64 waves: below one wave per SIMD (256 CUs
+    // x 4), one wave per point instead (each wave then reads every source
+    // record, which is cheap only while the points are few)
+    if (nf < kRowsBelow) {
+        dim3 grid((unsigned)((nf + 3) / 4)), block(256);
+        hipLaunchKernelGGL((k_contract_rows<KIND>), grid, block, 0, st, xf, nf, src, lds, ns, dens, cf, mu, lam, out, scale, beta);
+    } else {
+        dim3 grid((unsigned)((nf + 255) / 256)), block(256);
+        hipLaunchKernelGGL((k_contract<KIND>), grid, block, 0, st, xf, nf, src, lds, ns, dens, cf, mu, lam, out, scale, beta);
+    }
     return launch_status();
 }
 
