@@ -167,9 +167,19 @@ def upload(a, dtype):
     return t.to(device(), non_blocking=True)
 
 
+_raw_stream = None
+
+
 def stream():
-    """hipStream_t of torch's current stream on the current device."""
-    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    """hipStream_t of torch's current stream on the current device (the raw
+    handle torch's own graph tracer reads; the first call goes through
+    ``torch.cuda.current_stream`` so the runtime is initialised)."""
+    global _raw_stream
+    if _raw_stream is None:
+        torch.cuda.current_stream()
+        get, dev = torch._C._cuda_getCurrentRawStream, torch._C._cuda_getDevice
+        _raw_stream = lambda: get(dev())  # noqa: E731
+    return ctypes.c_void_p(_raw_stream())
 
 
 def ptr(t) -> ctypes.c_void_p:

@@ -10,15 +10,23 @@ from .. import defaults
 
 
 class ClusterNode:
-    __slots__ = ("indices", "lo", "hi", "left", "right")
+    """A node: its element indices and children; the bounding box (lo, hi)
+    is computed when read (the block-Jacobi chunks need only the leaves)."""
+    __slots__ = ("indices", "_centroids", "left", "right")
 
     def __init__(self, indices, centroids):
         self.indices = indices
-        pts = centroids[indices]
-        self.lo = pts.min(axis=0)
-        self.hi = pts.max(axis=0)
+        self._centroids = centroids
         self.left = None
         self.right = None
+
+    @property
+    def lo(self):
+        return self._centroids[self.indices].min(axis=0)
+
+    @property
+    def hi(self):
+        return self._centroids[self.indices].max(axis=0)
 
     @property
     def is_leaf(self) -> bool:
@@ -39,14 +47,13 @@ def build_cluster_tree(centroids: np.ndarray, min_leaf: int = defaults.CLUSTER_M
         node = ClusterNode(idx, centroids)
         if len(idx) <= min_leaf:
             return node
-        pts = centroids[idx]
-        mean = pts.mean(axis=0)
-        cov = (pts - mean).T @ (pts - mean)
-        _, vecs = np.linalg.eigh(cov)
-        proj = (pts - mean) @ vecs[:, -1]
-        median = np.median(proj)
-        left = proj < median
-        if left.sum() == 0 or (~left).sum() == 0:
+        c = centroids[idx]
+        c = c - c.mean(axis=0)
+        _, vecs = np.linalg.eigh(c.T @ c)
+        proj = c @ vecs[:, -1]
+        left = proj < np.median(proj)
+        nl = int(np.count_nonzero(left))
+        if nl == 0 or nl == len(idx):
             return node
         node.left = _build(idx[left])
         node.right = _build(idx[~left])

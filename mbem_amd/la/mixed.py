@@ -50,24 +50,45 @@ def near_inputs(assembled, k_near: float = 2.0):
     return pts, cen, rad
 
 
+class NearPattern:
+    """The near-field block pattern in two phases: the constructor uploads
+    the geometry (pinned, non-blocking) and queues the counting pass on the
+    current stream; ``result()`` reads the counts, queues the filling pass
+    and returns (rowptr, col) as host arrays.  A caller can do other host
+    work between the two while the counts are computed."""
+
+    def __init__(self, assembled, k_near: float = 2.0):
+        pts, cen, rad = near_inputs(assembled, k_near)
+        self.nb = pts.shape[0]
+        self.P, self.C, self.Rd = (_lib.upload(a, F64) for a in (pts, cen, rad))
+        self.stream = torch.cuda.current_stream()
+        self.counts = torch.empty(self.nb, dtype=I64, device=self.P.device)
+        call("mbem_near_pattern", ptr(self.P), ptr(self.C), ptr(self.Rd), self.nb,
+             ptr(self.counts), None, None, stream())
+        self._out = None
+
+    def result(self) -> tuple:
+        if self._out is None:
+            with torch.cuda.stream(self.stream):
+                nb, dev = self.nb, self.P.device
+                rowptr = torch.zeros(nb + 1, dtype=I64, device=dev)
+                rowptr[1:] = torch.cumsum(self.counts, 0)
+                nnz = int(rowptr[-1].item())
+                col = torch.empty(max(nnz, 1), dtype=I64, device=dev)
+                call("mbem_near_pattern", ptr(self.P), ptr(self.C), ptr(self.Rd), nb, None,
+                     ptr(rowptr), ptr(col), stream())
+                self._out = (rowptr.cpu().numpy(), col[:nnz].cpu().numpy())
+            self.P = self.C = self.Rd = self.counts = None
+        return self._out
+
+
 def near_structure(assembled, k_near: float = 2.0):
     """Block-CSR pattern (rowptr, col) of the near field over the system's
     unknown blocks (row block i = collocation point of unknown triple i,
     column block j = source element of unknown triple j): near when
     |x_i - c_j| < k_near * (eps_j + h_j).  Counted and filled on the GPU
     (mbem_near_pattern); returned as host arrays."""
-    pts, cen, rad = near_inputs(assembled, k_near)
-    dev = assembled.A_dev.device
-    nb = pts.shape[0]
-    P, C, Rd = (torch.as_tensor(a, dtype=F64, device=dev) for a in (pts, cen, rad))
-    counts = torch.empty(nb, dtype=I64, device=dev)
-    call("mbem_near_pattern", ptr(P), ptr(C), ptr(Rd), nb, ptr(counts), None, None, stream())
-    rowptr = torch.zeros(nb + 1, dtype=I64, device=dev)
-    rowptr[1:] = torch.cumsum(counts, 0)
-    nnz = int(rowptr[-1].item())
-    col = torch.empty(max(nnz, 1), dtype=I64, device=dev)
-    call("mbem_near_pattern", ptr(P), ptr(C), ptr(Rd), nb, None, ptr(rowptr), ptr(col), stream())
-    return rowptr.cpu().numpy(), col[:nnz].cpu().numpy()
+    return NearPattern(assembled, k_near).result()
 
 
 class MixedOperator:
@@ -345,8 +366,8 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     solves (``_RecycledInner``)."""
     from .preconditioner import BlockGaussSeidel
     # Two streams (both at the default priority).  The near pattern needs
-    # the geometry only: its small kernels run on a side stream beside the
-    # assembly.  The block-Jacobi build's host work (cluster tree, launches)
+    # the geometry only: its counting pass is queued on a side stream beside
+    # the assembly and read back after the block-Jacobi host work.  The block-Jacobi build's host work (cluster tree, launches)
     # runs while the assembly occupies the device.  Once its gathers have
     # read A (``gathered``), the operator build goes on the side stream (near
     # blocks held out of A, int8 cast, blocks restored), queued before the
@@ -355,16 +376,17 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     t0 = time.perf_counter()
     main = torch.cuda.current_stream()
     side = torch.cuda.Stream()
+    near = None
     if pattern is None:
         with torch.cuda.stream(side):
-            pattern = near_structure(assembled, k_near)
+            near = NearPattern(assembled, k_near)     # counting pass queued, not waited for
     built = {}
 
     def start_operator(gathered):
+        pat = near.result() if near is not None else pattern
         side.wait_event(gathered)
         with torch.cuda.stream(side):
-            built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near,
-                                        pattern=pattern)
+            built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pat)
 
     M = BlockGaussSeidel(assembled, max_dense=precond_max_dense, bj_chunk=precond_bj_chunk,
                          grouping=precond_grouping, inverse_dtype="f32", defer_check=True,
