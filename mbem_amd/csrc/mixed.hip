@@ -626,7 +626,7 @@ int mbem_gemv_i8(const uint8_t* F, int64_t ldf, const float* scale, int64_t nseg
     constexpr int R = MBEM_I8_ROWS_PER_WAVE;
     long waves = (m + R - 1) / R;
     dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-    hipLaunchKernelGGL((k_gemv_i8<R, 4>), grid, block, 0, stream, F, ldf, scale, nseg, m, n, x, y, alpha, beta);
+    hipLaunchKernelGGL((k_gemv_i8<R, MBEM_I8_LOADS_PER_ROW>), grid, block, 0, stream, F, ldf, scale, nseg, m, n, x, y, alpha, beta);
     return launch_status();
 }
 
