@@ -39,6 +39,10 @@ extern "C" {
 /* rows of a block-Jacobi chunk one wavefront applies (mbem_bj_apply tables) */
 #define MBEM_BJ_ROWS_PER_WAVE 4
 
+/* rows of a padded FP32 block-Jacobi chunk one wavefront applies
+ * (mbem_bj_apply_f32v tables) */
+#define MBEM_BJ32_ROWS_PER_WAVE 8
+
 /* rows of the int8 far field one wavefront multiplies (mbem_gemv_i8), and
  * the 16-byte loads per row each lane keeps in flight (measured at n = 60,000:
  * 8 x 2 623 us, 8 x 3 625 us, 4 x 4 644 us, 4 x 3 652 us) */
@@ -283,6 +287,19 @@ int mbem_invert_batched_f32(float *M, int64_t m, int64_t batch, int32_t *ipiv, i
 int mbem_bj_apply_f32(const int64_t *wave_table, int64_t nwaves, const int64_t *chunk_ptr,
                       const int64_t *idx, const int64_t *inv_ptr, const float *inv, const double *r,
                       double *z, hipStream_t stream);
+
+/* FP32 block-Jacobi on padded rows (the form la/preconditioner.py uses):
+ * chunk c holds m = chunk_ptr[c+1] - chunk_ptr[c] DOFs idx[chunk_ptr[c]..],
+ * its inverse at inv + inv_ptr[c] with rows of ldm = pad_ptr[c+1] - pad_ptr[c]
+ * floats (a multiple of 4, pad columns zero).  mbem_gather_f32 fills rr[p] =
+ * r[gidx[p]] (0 where gidx[p] < 0: the pads) in that padded chunk order;
+ * mbem_bj_apply_f32v then sets z[idx[i]] = sum_j inv_c[i, j] rr_c[j] for the
+ * (chunk, first row) pairs of wave_table, MBEM_BJ32_ROWS_PER_WAVE rows each.
+ * inv and rr 16-byte aligned. */
+int mbem_gather_f32(const int64_t *gidx, int64_t np, const double *r, float *rr, hipStream_t stream);
+int mbem_bj_apply_f32v(const int64_t *wave_table, int64_t nwaves, const int64_t *chunk_ptr,
+                       const int64_t *pad_ptr, const int64_t *idx, const int64_t *inv_ptr,
+                       const float *inv, const float *rr, double *z, hipStream_t stream);
 
 /* ---- Batched Laplace-state solves (rheology.py:479-524, dense.py:650-687) ----
  * The reference solves its Laplace states one after another: a re-assembly

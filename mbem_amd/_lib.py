@@ -19,6 +19,7 @@ LIB_PATH = os.path.join(_HERE, "libmbem_hip.so")
 
 MBEM_OK, MBEM_EINVAL, MBEM_EHIP = 0, 1, 2
 BJ_ROWS_PER_WAVE = 4
+BJ32_ROWS_PER_WAVE = 8
 
 _p = ctypes.c_void_p
 _i64 = ctypes.c_int64
@@ -73,6 +74,8 @@ _SIGS = {
     "mbem_gather_block_f32": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert_batched_f32": [_p, _i64, _i64, _p, _p, _p],
     "mbem_bj_apply_f32": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
+    "mbem_gather_f32": [_p, _i64, _p, _p, _p],
+    "mbem_bj_apply_f32v": [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
     "mbem_src_fields": [],
     "mbem_multidot_work": [_i64, _i64],
     # batched Laplace-state solves

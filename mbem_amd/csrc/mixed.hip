@@ -470,6 +470,64 @@ __global__ __launch_bounds__(256) void k_bj_apply_f32(const int64_t* __restrict_
     }
 }
 
+// FP32 block-Jacobi on padded rows: chunk c's inverse is stored with its m
+// rows padded to ldm = pptr[c+1] - pptr[c] (a multiple of 4, pads zero), and
+// the right-hand side is gathered once per apply into the same padded chunk
+// order in FP32 (k_gather_f32), so each lane reads 16-byte float4s of both;
+// the products accumulate in FP64.  One wave per R rows of one chunk.
+__global__ __launch_bounds__(256) void k_gather_f32(const int64_t* __restrict__ gidx, long np,
+                                                    const double* __restrict__ r,
+                                                    float* __restrict__ rr)
+{
+    const long p = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= np) return;
+    const int64_t g = gidx[p];
+    rr[p] = (g >= 0) ? (float)r[g] : 0.0f;
+}
+
+template <int R>
+__global__ __launch_bounds__(256) void k_bj_apply_f32v(const int64_t* __restrict__ wtab, long nwaves,
+                                                       const int64_t* __restrict__ cptr,
+                                                       const int64_t* __restrict__ pptr,
+                                                       const int64_t* __restrict__ idx,
+                                                       const int64_t* __restrict__ iptr,
+                                                       const float* __restrict__ inv,
+                                                       const float* __restrict__ rr,
+                                                       double* __restrict__ z)
+{
+    const int lane = threadIdx.x & 63;
+    const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (wave >= nwaves) return;
+    const long c = wtab[2 * wave], row0 = wtab[2 * wave + 1];
+    const long m = cptr[c + 1] - cptr[c];
+    const long ldm = pptr[c + 1] - pptr[c];
+    const flt4* x = reinterpret_cast<const flt4*>(rr + pptr[c]);
+    const flt4* B = reinterpret_cast<const flt4*>(inv + iptr[c]);
+    const long n4 = ldm / 4;
+    double acc[R];
+#pragma unroll
+    for (int q = 0; q < R; ++q) acc[q] = 0.0;
+    for (long j = lane; j < n4; j += 64) {
+        const flt4 xv = x[j];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const long i = row0 + q;
+            if (i < m) {
+                const flt4 b = __builtin_nontemporal_load(B + i * n4 + j);
+                acc[q] += (double)b.x * (double)xv.x + (double)b.y * (double)xv.y +
+                          (double)b.z * (double)xv.z + (double)b.w * (double)xv.w;
+            }
+        }
+    }
+    const int64_t* id = idx + cptr[c];
+#pragma unroll
+    for (int q = 0; q < R; ++q) {
+        double t = wave_sum_d(acc[q]);
+        long i = row0 + q;
+        if (lane == 0 && i < m) z[id[i]] = t;
+    }
+}
+
 // near-field pattern: row block i is near source block j when
 // |pts_i - cen_j|^2 < rad_j^2.  One wave per row block; lanes scan the
 // sources 64 at a time and a ballot keeps the columns in ascending order.
@@ -664,6 +722,28 @@ int mbem_invert_batched_f32(float* M, int64_t m, int64_t batch, int32_t* ipiv, i
         return MBEM_EHIP;
     if (rocsolver_sgetri_strided_batched(g_handle32, mm, M, mm, sA, ipiv, sP, info, (rocblas_int)batch) != rocblas_status_success)
         return MBEM_EHIP;
+    return launch_status();
+}
+
+int mbem_gather_f32(const int64_t* gidx, int64_t np, const double* r, float* rr, hipStream_t stream)
+{
+    if (np < 0 || (np > 0 && (!gidx || !r || !rr))) return MBEM_EINVAL;
+    if (np == 0) return MBEM_OK;
+    hipLaunchKernelGGL(k_gather_f32, dim3((unsigned)((np + 255) / 256)), dim3(256), 0, stream, gidx, np, r, rr);
+    return launch_status();
+}
+
+int mbem_bj_apply_f32v(const int64_t* wave_table, int64_t nwaves, const int64_t* chunk_ptr,
+                       const int64_t* pad_ptr, const int64_t* idx, const int64_t* inv_ptr,
+                       const float* inv, const float* rr, double* z, hipStream_t stream)
+{
+    if (nwaves < 0 || (nwaves > 0 && (!wave_table || !chunk_ptr || !pad_ptr || !idx || !inv_ptr ||
+                                      !inv || !rr || !z)))
+        return MBEM_EINVAL;
+    if (nwaves == 0) return MBEM_OK;
+    if ((reinterpret_cast<uintptr_t>(inv) & 15) || (reinterpret_cast<uintptr_t>(rr) & 15)) return MBEM_EINVAL;
+    constexpr int R = MBEM_BJ32_ROWS_PER_WAVE;
+    hipLaunchKernelGGL((k_bj_apply_f32v<R>), dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, stream, wave_table, nwaves, chunk_ptr, pad_ptr, idx, inv_ptr, inv, rr, z);
     return launch_status();
 }
 

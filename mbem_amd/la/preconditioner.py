@@ -193,16 +193,41 @@ class BlockGaussSeidel:
         # device tables: chunk DOF lists, inverse offsets, per-block wave tables
         sizes = np.array([c[1].size for c in chunks], dtype=np.int64)
         cptr = np.concatenate([[0], np.cumsum(sizes)])
-        iptr = np.concatenate([[0], np.cumsum(sizes * sizes)])
+        f32 = inverse_dtype == "f32"
+        # FP32 inverses keep their rows padded to a multiple of 4 floats
+        # (16-byte loads; pads zero) and gather r once per apply into the
+        # same padded chunk order (gidx: the DOF of each padded slot, -1 for
+        # a pad); FP64 inverses are packed m x m
+        ld = ((sizes + 3) // 4) * 4 if f32 else sizes
+        iptr = np.concatenate([[0], np.cumsum(sizes * ld)])
         # pinned, non-blocking uploads: the host goes on to queue the
         # inversions behind whatever the stream still runs (the assembly)
         self.chunk_ptr = _lib.upload(cptr, I64)
         self.inv_ptr = _lib.upload(iptr, I64)
         self.idx = _lib.upload(np.concatenate([c[1] for c in chunks]), I64)
-        f32 = inverse_dtype == "f32"
-        self.inv = torch.empty(int(iptr[-1]), dtype=torch.float32 if f32 else F64, device=dev)
-        R = _lib.BJ_ROWS_PER_WAVE
+        if f32:
+            pptr = np.concatenate([[0], np.cumsum(ld)])
+            gidx = np.full(int(pptr[-1]), -1, dtype=np.int64)
+            for ci, (_, dofs) in enumerate(chunks):
+                gidx[pptr[ci]:pptr[ci] + dofs.size] = dofs
+            self.pad_ptr = _lib.upload(pptr, I64)
+            self.gidx = _lib.upload(gidx, I64)
+            self.rr32 = torch.zeros(int(pptr[-1]), dtype=torch.float32, device=dev)
+            self.inv = torch.zeros(int(iptr[-1]), dtype=torch.float32, device=dev)
+        else:
+            self.inv = torch.empty(int(iptr[-1]), dtype=F64, device=dev)
+        self._ld = ld
+        R = _lib.BJ32_ROWS_PER_WAVE if f32 else _lib.BJ_ROWS_PER_WAVE
         tables = {k: [] for k in range(len(self.sbs))}
+        first = {}
+        for ci, (k, _) in enumerate(chunks):
+            first.setdefault(k, ci)
+        for k, sb in enumerate(self.sbs):
+            c0 = first.get(k, len(chunks))
+            c1 = c0 + sum(1 for c in chunks if c[0] == k)
+            sb.chunks = (c0, c1)
+            if f32:
+                sb.pad = (int(pptr[c0]), int(pptr[c1]))
         for ci, (k, dofs) in enumerate(chunks):
             tables[k] += [(ci, r0) for r0 in range(0, int(dofs.size), R)]
         # invert the chunks in size classes (largest / smallest <= 1.1): each
@@ -250,8 +275,8 @@ class BlockGaussSeidel:
             self._pending.append((info, cls, sizes[cls]))
             for q, ci in enumerate(cls):
                 m = int(sizes[ci])
-                call("mbem_copy2d", ptr(out[q]), mp, ptr(self.inv[int(iptr[ci]):]), m, m, m, esz,
-                     stream())
+                call("mbem_copy2d", ptr(out[q]), mp, ptr(self.inv[int(iptr[ci]):]), int(ld[ci]), m,
+                     m, esz, stream())
         del classes
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
@@ -262,7 +287,7 @@ class BlockGaussSeidel:
         if not defer_check:
             self.finish()
         self.n_chunks = len(chunks)
-        self.inverse_bytes = int(iptr[-1]) * (4 if f32 else 8)
+        self.inverse_bytes = int(iptr[-1]) * (4 if f32 else 8)     # padding included
         self._rr = torch.empty(n, dtype=F64, device=dev)
 
     def finish(self) -> "BlockGaussSeidel":
@@ -295,9 +320,7 @@ class BlockGaussSeidel:
             seg = slice(sb.start, sb.stop)
             if sb.start == 0:
                 # the first super-block has no lower coupling: read r itself
-                call("mbem_bj_apply_f32" if self.inverse_dtype == "f32" else "mbem_bj_apply",
-                     ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr), ptr(self.idx),
-                     ptr(self.inv_ptr), ptr(self.inv), ptr(r), ptr(z), stream())
+                self._apply_block(sb, r, z)
                 continue
             rr[seg].copy_(r[seg])
             if sb.start > 0:
@@ -306,10 +329,21 @@ class BlockGaussSeidel:
                 else:
                     call("mbem_gemv", ptr(A[sb.start:sb.stop]), A.stride(0), sb.size, sb.start,
                          ptr(z), ptr(rr[seg]), -1.0, 1.0, stream())
-            call("mbem_bj_apply_f32" if self.inverse_dtype == "f32" else "mbem_bj_apply",
-                 ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr), ptr(self.idx),
-                 ptr(self.inv_ptr), ptr(self.inv), ptr(rr), ptr(z), stream())
+            self._apply_block(sb, rr, z)
         return z
+
+    def _apply_block(self, sb, r, z):
+        """z[sb's DOFs] = (its chunk inverses) r[sb's DOFs]."""
+        if self.inverse_dtype == "f32":
+            p0, p1 = sb.pad
+            call("mbem_gather_f32", ptr(self.gidx[p0:p1]), p1 - p0, ptr(r), ptr(self.rr32[p0:p1]),
+                 stream())
+            call("mbem_bj_apply_f32v", ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr),
+                 ptr(self.pad_ptr), ptr(self.idx), ptr(self.inv_ptr), ptr(self.inv),
+                 ptr(self.rr32), ptr(z), stream())
+        else:
+            call("mbem_bj_apply", ptr(sb.wave_table), sb.n_waves, ptr(self.chunk_ptr),
+                 ptr(self.idx), ptr(self.inv_ptr), ptr(self.inv), ptr(r), ptr(z), stream())
 
     def __call__(self, r: torch.Tensor) -> torch.Tensor:
         z = torch.empty_like(r)
@@ -324,7 +358,8 @@ class BlockGaussSeidel:
         iptr = self.inv_ptr.cpu().numpy()
         for c in range(len(cptr) - 1):
             m = int(cptr[c + 1] - cptr[c])
-            M = self.inv[int(iptr[c]):int(iptr[c]) + m * m].view(m, m)
+            ld = int(self._ld[c])
+            M = self.inv[int(iptr[c]):int(iptr[c]) + m * ld].view(m, ld)[:, :m]
             d[self.idx[int(cptr[c]):int(cptr[c + 1])]] = M.diagonal().to(F64)
         return d
 
