@@ -184,7 +184,7 @@ def test_trained_network_ab_on_a_held_out_geometry():
     """The committed weights (mbem_amd/data/learned_precond.npz) against
     block-Jacobi alone on a geometry outside the training draw: both solves
     converge to the same solution; the iteration ratio is printed (measured
-    1.23-1.37x more iterations with the learned correction on the 25 held-out
+    1.08-1.47x more iterations with the learned correction on the 50 held-out
     training-family geometries: the network does not help) and kept within
     [0.5, 2] so a broken network or apply shows up."""
     from mbem_amd.backends.dense import AssembledDense
