@@ -247,3 +247,21 @@ def evaluate_stress(model: RegionModel, region, solution: dict, points, eps,
     torch.cuda.current_stream().synchronize()
     s = sig.cpu().numpy()
     return (s, eig.cpu().numpy()) if parts else s
+
+
+def evaluate_fault_traction(model: RegionModel, region, solution: dict, fault, eps,
+                            subtract_anelastic: bool = True) -> np.ndarray:
+    """Elastic traction (n_dofs, 3) at a fault's collocation points: the
+    ``evaluate_stress`` field there dotted with the collocation normals
+    (evaluate.py:475-492).  ``fault`` is a fault of ``region`` or its name.
+    On a prescribed-slip fault this is the on-fault stress readout of the
+    slip."""
+    region = _region(model, region)
+    if isinstance(fault, str):
+        fault = next((f for f in region.faults if f.name == fault), None)
+        if fault is None:
+            raise KeyError(f"region {region.name!r} has no fault of that name")
+    sig = evaluate_stress(model, region, solution, fault.collocation_points(), eps,
+                          subtract_anelastic=subtract_anelastic, warn_near=False)
+    return np.einsum("nij,nj->ni", sig, fault.collocation_normals())
+

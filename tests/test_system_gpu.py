@@ -156,6 +156,30 @@ def test_readout(c1):
     assert np.max(np.abs(s_full - sr_full)) < 1e-10 * np.max(np.abs(sr_full))
 
 
+def test_fault_traction(c1):
+    """evaluate_fault_traction (evaluate.py:475-492): the stress readout at the
+    fault's collocation points dotted with their normals, against the oracle's
+    stress at the same points (1e-10 of the largest entry); the fault may be
+    named; an unknown name raises."""
+    from mbem_amd.evaluate import evaluate_fault_traction
+    from oracle import evaluate as oe
+    from oracle.dense import split
+    model = c1["model"]
+    region = model.regions[0]
+    fault = region.faults[0]
+    sol = split(c1["system"].layout, c1["x_lu"])
+    eps = c1["eps"]
+    t = evaluate_fault_traction(model, region, sol, fault, eps)
+    sr = oe.evaluate_stress(model, region, sol, fault.collocation_points(), eps)
+    tr = np.einsum("nij,nj->ni", sr, fault.collocation_normals())
+    print(f"fault traction rel err {_rel(t, tr):.2e}")
+    assert t.shape == (fault.collocation_points().shape[0], 3)
+    assert np.max(np.abs(t - tr)) < 1e-10 * np.max(np.abs(tr))
+    assert np.array_equal(evaluate_fault_traction(model, region.name, sol, fault.name, eps), t)
+    with pytest.raises(KeyError):
+        evaluate_fault_traction(model, region, sol, "no such fault", eps)
+
+
 def test_rigid_translation_gives_zero_traction():
     """Manufactured solution (verify_solved_bvp.py B1): prescribe a constant
     displacement on every wall and the base, leave the top free: the solve
