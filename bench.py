@@ -35,6 +35,7 @@ sys.path.insert(0, ROOT)
 METRIC = "solve time to 1e-8 rel err (N=20k tris): mixed+learned vs plain FP64 GPU"
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3                # dense FP32 matrix-core peak (sgetrf roofline)
 
 
 def _measured_traffic_ratio(kernel: str = "k_gemv"):
@@ -458,7 +459,8 @@ def bench_lu(args):
     model, eps, info = box_fault_model(args.n_cells, fault_cells=tuple(args.fault_cells))
     system = generate_system(model)
     n = system.layout.n_unknowns
-    asm = AssembledDense(system, eps, "direct", jump="calibrated")
+    how = args.lu_how
+    asm = AssembledDense(system, eps, "direct", jump="calibrated", lu=how)
     x_it = asm.solve(rtol=1e-13, precond_max_dense=0, precond_bj_chunk=args.bj_chunk,
                      return_device=True, precond_grouping=args.grouping).clone()
     asm.release()
@@ -473,7 +475,7 @@ def bench_lu(args):
         dt = time.perf_counter() - t0
         fs = asm.report.lu["factor_s"]
         print(f"[lu] {'warmup' if k < args.warmup else 'step'} {k}: {dt:.3f} s (getrf {fs:.3f} s, "
-              f"residual {asm.report.residual:.2e})", flush=True)
+              f"residual {asm.report.residual:.2e}; {asm.report.lu})", flush=True)
         if k >= args.warmup:
             times.append(dt)
             factor.append(fs)
@@ -494,17 +496,25 @@ def bench_lu(args):
                          f"of the assembled matrix ({t_m:.2f} s), scaled by (n/m)^3 to n = {n}"}
         print(f"[lu] cpu baseline {cpu['value']:.1f} s", flush=True)
     value = float(np.mean(times))
+    single = how != "double"
+    peak = FP32_PEAK_TFLOPS if single else FP64_PEAK_TFLOPS
     line = {
-        "metric": "dense LU solve time (N=20k tris, FP64)", "value": value, "unit": "s",
+        "metric": ("dense LU solve time (N=20k tris, FP32 factors + FP64 refinement)" if single
+                   else "dense LU solve time (N=20k tris, FP64)"), "value": value, "unit": "s",
         "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": 1e3 * value,
-        "higher_is_better": False, "scaling": "strong", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f32+f64" if single else "f64",
         "data": "synthetic box-with-dipping-fault mesh generated in bench.py",
         "config": {"workload": f"box 40x40x20 km, N={info['box_triangles']} P0 triangles (n={n})",
-                   "getrf_s": f_s, "rel_diff_vs_fgmres_1e-13": err,
-                   "residual": asm.report.residual},
-        "roofline": {"bound": "mfma", "achieved": flops / f_s / 1e12, "peak": FP64_PEAK_TFLOPS,
-                     "unit": "TFLOP/s", "frac": flops / f_s / 1e12 / FP64_PEAK_TFLOPS,
-                     "traffic": None, "kernel": "rocSOLVER dgetrf (2/3 n^3 flops)",
+                   "lu": how, "getrf_s": f_s, "rel_diff_vs_fgmres_1e-13": err,
+                   "residual": asm.report.residual,
+                   "refinement": {k: v for k, v in asm.report.lu.items()
+                                  if k in ("passes", "correction", "left", "fallback")}},
+        "roofline": {"bound": "mfma", "achieved": flops / f_s / 1e12, "peak": peak,
+                     "unit": "TFLOP/s", "frac": flops / f_s / 1e12 / peak,
+                     "traffic": None,
+                     "kernel": ("rocSOLVER sgetrf" if single else "rocSOLVER dgetrf")
+                               + " (2/3 n^3 flops; factor_s includes the FP32 copy)",
                      "avg_ms": 1e3 * f_s},
         "cpu_baseline": cpu,
     }
@@ -535,6 +545,8 @@ def main():
                          "batched Laplace-state solves; readout: stress readout throughput; "
                          "lu: the dense LU solve")
     ap.add_argument("--lu-cpu-block", type=int, default=6000)
+    ap.add_argument("--lu-how", choices=("double", "single", "gpu"), default="double",
+                    help="LU flavour of --workload lu (single/gpu: FP32 factors, FP64 refinement)")
     ap.add_argument("--readout-points", type=int, default=1_000_000)
     ap.add_argument("--readout-cpu-points", type=int, default=200)
     ap.add_argument("--laplace-cells", type=int, default=36)

@@ -199,6 +199,17 @@ int mbem_lu_solve_blocked(const double *LU, int64_t n, const int64_t *perm, cons
                           const double *inv_u, const double *b, double *x, double *work,
                           int adjoint, hipStream_t stream);
 
+/* The same in single precision for lu="single"/"gpu" (la/lu.py:119-239:
+ * single-precision factors, every solve refined in double by the caller):
+ * sgetrf of the row-major FP32 copy (n x n, in place), the diagonal-block
+ * inverses formed in FP64 from the FP32 factors, and the blocked solves
+ * reading FP32 factors with FP64 vectors and accumulation. */
+int mbem_lu_factor_f32(float *A, int64_t n, int64_t *ipiv, int64_t *info, hipStream_t stream);
+int mbem_lu_prepare_f32(const float *LU, int64_t n, double *inv_l, double *inv_u, hipStream_t stream);
+int mbem_lu_solve_blocked_f32(const float *LU, int64_t n, const int64_t *perm,
+                              const double *inv_l, const double *inv_u, const double *b,
+                              double *x, double *work, int adjoint, hipStream_t stream);
+
 /* A^T x = b with the factors of mbem_lu_factor, in place, nrhs columns: the
  * transposed solves of the 1-norm condition estimate that LAPACK's gecon
  * makes on the reference's factors (la/lu.py:241-248). */

@@ -58,6 +58,9 @@ _SIGS = {
     "mbem_lu_inv_doubles": [_i64],
     "mbem_lu_prepare": [_p, _i64, _p, _p, _p],
     "mbem_lu_solve_blocked": [_p, _i64, _p, _p, _p, _p, _p, _p, _int, _p],
+    "mbem_lu_factor_f32": [_p, _i64, _p, _p, _p],
+    "mbem_lu_prepare_f32": [_p, _i64, _p, _p, _p],
+    "mbem_lu_solve_blocked_f32": [_p, _i64, _p, _p, _p, _p, _p, _p, _int, _p],
     "mbem_colsum_work": [_i64],
     "mbem_colsum_abs": [_p, _i64, _i64, _i64, _p, _p, _p],
     "mbem_bj_apply": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],

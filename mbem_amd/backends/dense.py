@@ -45,7 +45,9 @@ F64 = torch.float64
 class DenseBackend:
     """dense.py:49-86.  ``jump="calibrated"`` annihilates constant fields
     exactly; ``deflate`` borders out the rigid translations of an all-Neumann
-    model; ``lu`` names the LU flavour of ``solve(method="lu")`` ("double")."""
+    model; ``lu`` names the LU flavour of ``solve(method="lu")``: "double"
+    (the default), or "single" / "gpu" (la/lu.py:1-36: single-precision
+    factors, every solve refined in double; both on the device here)."""
 
     def __init__(self, mode: str = "direct", jump: str = "calibrated", deflate: bool = False,
                  lu: str | None = None, calibrate_all_rows: bool = False):
@@ -56,8 +58,8 @@ class DenseBackend:
                                       f"GPU in this build")
         if jump not in ("half", "calibrated"):
             raise ValueError(jump)
-        if lu not in (None, "double"):
-            raise NotImplementedError(f"lu={lu!r}: only FP64 LU ('double') is in this build")
+        if lu not in (None,) + LU_HOW:
+            raise ValueError(f"a dense matrix is factored {LU_HOW}, not {lu!r}")
         self.mode = mode
         self.jump = jump
         self.deflate = deflate
@@ -173,18 +175,24 @@ class _Lineage:
         self.recycle_materials = None
 
 
+LU_HOW = ("double", "single", "gpu")      # la/lu.py:49
+
+
 class _LUFactors:
-    """Device LU factors of a row-major matrix M (mbem_lu_factor) with what
-    the blocked solves need: the inverses of the 64 x 64 diagonal blocks of
-    L and U and the pivots as a row order (P^T v = v[perm])."""
+    """Device LU factors of a row-major matrix M (mbem_lu_factor, or
+    mbem_lu_factor_f32 for an FP32 copy) with what the blocked solves need:
+    the inverses of the 64 x 64 diagonal blocks of L and U and the pivots as
+    a row order (P^T v = v[perm])."""
 
     def __init__(self, LU: torch.Tensor, ipiv: torch.Tensor, m: int):
         self.LU, self.ipiv, self.m = LU, ipiv, m
+        self.f32 = LU.dtype == torch.float32
         dev = LU.device
         k = int(_lib.load().mbem_lu_inv_doubles(m))
         self.inv_l = torch.empty(max(k, 1), dtype=F64, device=dev)
         self.inv_u = torch.empty(max(k, 1), dtype=F64, device=dev)
-        call("mbem_lu_prepare", ptr(LU), m, ptr(self.inv_l), ptr(self.inv_u), stream())
+        call("mbem_lu_prepare_f32" if self.f32 else "mbem_lu_prepare", ptr(LU), m,
+             ptr(self.inv_l), ptr(self.inv_u), stream())
         piv = ipiv.cpu().numpy() - 1            # rocSOLVER pivots are 1-based
         perm = np.arange(m, dtype=np.int64)
         for i, p in enumerate(piv.tolist()):
@@ -195,9 +203,45 @@ class _LUFactors:
 
     def solve_into(self, b: torch.Tensor, x: torch.Tensor, adjoint: bool = False):
         """x = M^-1 b (adjoint: M^-T b); b and x may be the same vector."""
-        call("mbem_lu_solve_blocked", ptr(self.LU), self.m, ptr(self.perm), ptr(self.inv_l),
-             ptr(self.inv_u), ptr(b), ptr(x), ptr(self.work), int(adjoint), stream())
+        call("mbem_lu_solve_blocked_f32" if self.f32 else "mbem_lu_solve_blocked", ptr(self.LU),
+             self.m, ptr(self.perm), ptr(self.inv_l), ptr(self.inv_u), ptr(b), ptr(x),
+             ptr(self.work), int(adjoint), stream())
         return x
+
+
+def refine(solve_into, matvec_into, b: torch.Tensor, tol: float = None, passes: int = None,
+           max_rate: float = None):
+    """Classical refinement of a single-precision solve (la/lu.py:198-239):
+    x = solve(b), then x += solve(b - M x) until what is left of the error,
+    rate / (1 - rate) |d| / |x| with rate = |d| / |d_prev| (the first
+    correction against the solution itself), or the correction itself, is
+    at most ``tol``; at most ``passes`` corrections; stop early when a
+    correction contracts slower than ``max_rate``.  Returns (x, report) with
+    report = {done, passes, correction, left, rate}."""
+    tol = defaults.LU_REFINE_TOL if tol is None else tol
+    passes = defaults.LU_REFINE_PASSES if passes is None else passes
+    max_rate = defaults.LU_REFINE_RATE if max_rate is None else max_rate
+    x = torch.empty_like(b)
+    solve_into(b, x)
+    xn = float(torch.linalg.vector_norm(x))
+    done = xn == 0.0
+    r = torch.empty_like(b)
+    d = torch.empty_like(b)
+    before, rate, size, left, k = None, 0.0, 0.0, 0.0, 0
+    while not done and k < passes:
+        matvec_into(x, r)                          # r = M x
+        torch.sub(b, r, out=r)                     # r = b - M x
+        solve_into(r, d)
+        x += d
+        k += 1
+        now = float(torch.linalg.vector_norm(d)) / float(torch.linalg.vector_norm(x))
+        rate = now if before is None else (now / before if before > 0.0 else 0.0)
+        size, before = now, now
+        left = rate / (1.0 - rate) * size if rate < 1.0 else float("inf")
+        done = left <= tol or size <= tol
+        if rate > max_rate:
+            break
+    return x, {"done": bool(done), "passes": k, "correction": size, "left": left, "rate": rate}
 
 
 @dataclass
@@ -634,29 +678,67 @@ class AssembledDense:
                 K[n:, :n] = Zd.T
                 LU = K
             else:
-                LU = self.A_dev.clone()
+                LU = self.A_dev
             m = LU.shape[0]
-            ipiv = torch.empty(m, dtype=torch.int64, device=dev)
-            info = torch.zeros(1, dtype=torch.int64, device=dev)
+            how = self.lu or defaults.DENSE_LU
             t0 = time.perf_counter()
             anorm = self._norm1(LU)
-            call("mbem_lu_factor", ptr(LU), m, ptr(ipiv), ptr(info), stream())
-            if int(info.item()) != 0:
-                raise np.linalg.LinAlgError(f"dense LU: singular (getrf info {int(info.item())})")
+            if how == "double":
+                if LU is self.A_dev:
+                    LU = LU.clone()                # getrf overwrites; A stays for the residual
+                fac = self._factor(LU, "mbem_lu_factor")
+            else:
+                # single-precision factors of an FP32 copy; the FP64 matrix is
+                # kept (only read) for the refinement's residuals
+                fac = self._factor(LU.to(torch.float32), "mbem_lu_factor_f32")
             fs = time.perf_counter() - t0
-            fac = _LUFactors(LU, ipiv, m)
-            self._lu = (fac, fs, self._cond1(fac, anorm))
-        fac, fs, cond = self._lu
+            self._lu = (fac, fs, self._cond1(fac, anorm), {"how": how, "factor_s": fs,
+                                                           "M": LU})
+        from ..la.solver import gemv
+        fac, fs, cond, lu_rep = self._lu
         m = fac.m
         rhs = torch.zeros(m, dtype=F64, device=dev)
         rhs[:n] = self.b_dev
-        fac.solve_into(rhs, rhs)
-        x = rhs[:n].contiguous()
+        if not fac.f32:
+            fac.solve_into(rhs, rhs)
+            y = rhs
+        else:
+            M = lu_rep["M"]
+            y, ref = refine(lambda v, out: fac.solve_into(v, out),
+                            lambda v, out: gemv(M, v, out), rhs)
+            lu_rep["passes"] = max(ref["passes"], lu_rep.get("passes", 0))
+            if ref["left"] >= lu_rep.get("left", 0.0):
+                lu_rep["correction"], lu_rep["left"] = ref["correction"], ref["left"]
+            if not ref["done"]:
+                warnings.warn(f"dense LU in single precision ({lu_rep['how']}): the refinement "
+                              f"did not contract (rate {ref['rate']:.1e}, {ref['left']:.1e} "
+                              f"left after {ref['passes']} corrections); factoring in double. "
+                              f"The matrix is too ill-conditioned for single-precision factors")
+                t0 = time.perf_counter()
+                fac = self._factor(M.clone(), "mbem_lu_factor")
+                lu_rep.update(fallback=True, factor_s=lu_rep["factor_s"] + time.perf_counter() - t0)
+                self._lu = (fac, fs, cond, lu_rep)
+                fac.solve_into(rhs, rhs)
+                y = rhs
+        x = y[:n].contiguous()
         r = self.b_dev - self.matvec(x)
         res = float(torch.linalg.vector_norm(r) / torch.linalg.vector_norm(self.b_dev))
         self.report = DenseSolveReport(cond_estimate=cond,
-                                       lu={"how": "double", "factor_s": fs}, residual=res)
+                                       lu={k: v for k, v in lu_rep.items() if k != "M"},
+                                       residual=res)
         return x
+
+    @staticmethod
+    def _factor(M: torch.Tensor, entry: str) -> "_LUFactors":
+        """getrf of M in place (FP64 or FP32 by ``entry``) and the blocked-solve
+        tables; raises on a singular factor."""
+        m = M.shape[0]
+        ipiv = torch.empty(m, dtype=torch.int64, device=M.device)
+        info = torch.zeros(1, dtype=torch.int64, device=M.device)
+        call(entry, ptr(M), m, ptr(ipiv), ptr(info), stream())
+        if int(info.item()) != 0:
+            raise np.linalg.LinAlgError(f"dense LU: singular (getrf info {int(info.item())})")
+        return _LUFactors(M, ipiv, m)
 
     @staticmethod
     def _norm1(M: torch.Tensor) -> float:

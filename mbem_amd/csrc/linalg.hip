@@ -291,30 +291,31 @@ constexpr int kTB = 64;
 
 // inverse of diagonal block blockIdx.x of L (LOWER, unit) or U, row-major
 // per block: inv[(k kTB + i) kTB + j]; a partial last block is padded with
-// the identity.  Thread t forms column t by substitution.
-template <bool LOWER>
-__global__ __launch_bounds__(kTB) void k_tri_inv_diag(const double* __restrict__ LU, long ld, long n,
+// the identity.  Thread t forms column t by substitution, in FP64 whatever
+// the factors' type T (FP64, or FP32 factors refined by the caller).
+template <typename T, bool LOWER>
+__global__ __launch_bounds__(kTB) void k_tri_inv_diag(const T* __restrict__ LU, long ld, long n,
                                                       double* __restrict__ inv)
 {
-    __shared__ double T[kTB][kTB + 1];
+    __shared__ double Tb[kTB][kTB + 1];
     __shared__ double Y[kTB][kTB + 1];
     const int t = threadIdx.x;
     const long k0 = (long)blockIdx.x * kTB;
     const int nb = (n - k0 < kTB) ? (int)(n - k0) : kTB;
     for (int i = 0; i < kTB; ++i)
-        T[i][t] = (i < nb && t < nb) ? LU[(k0 + i) + (k0 + t) * ld] : (i == t ? 1.0 : 0.0);
+        Tb[i][t] = (i < nb && t < nb) ? (double)LU[(k0 + i) + (k0 + t) * ld] : (i == t ? 1.0 : 0.0);
     __syncthreads();
     if (LOWER) {
         for (int i = 0; i < kTB; ++i) {
             double s = (i == t) ? 1.0 : 0.0;
-            for (int j = 0; j < i; ++j) s -= T[i][j] * Y[j][t];
+            for (int j = 0; j < i; ++j) s -= Tb[i][j] * Y[j][t];
             Y[i][t] = s;
         }
     } else {
         for (int i = kTB - 1; i >= 0; --i) {
             double s = (i == t) ? 1.0 : 0.0;
-            for (int j = i + 1; j < kTB; ++j) s -= T[i][j] * Y[j][t];
-            Y[i][t] = s / T[i][i];
+            for (int j = i + 1; j < kTB; ++j) s -= Tb[i][j] * Y[j][t];
+            Y[i][t] = s / Tb[i][i];
         }
     }
     for (int i = 0; i < kTB; ++i) inv[((long)blockIdx.x * kTB + i) * kTB + t] = Y[i][t];
@@ -325,8 +326,8 @@ __global__ __launch_bounds__(kTB) void k_tri_inv_diag(const double* __restrict__
 // T(i, j) = LU[i + j ld].  LOWER: the rows after K are updated (forward),
 // else the rows before K (backward).  Workgroup 0 writes x_K; the updated
 // rows are each owned by one workgroup.
-template <bool LOWER, bool ROWWISE>
-__global__ __launch_bounds__(256) void k_trsv_step(const double* __restrict__ LU, long ld, long n,
+template <typename T, bool LOWER, bool ROWWISE>
+__global__ __launch_bounds__(256) void k_trsv_step(const T* __restrict__ LU, long ld, long n,
                                                    const double* __restrict__ inv, long k,
                                                    double* __restrict__ r, double* __restrict__ x)
 {
@@ -357,9 +358,9 @@ __global__ __launch_bounds__(256) void k_trsv_step(const double* __restrict__ LU
     if (!ROWWISE) {
         const long i = lo + (long)blockIdx.x * 256 + t;
         if (i >= hi) return;
-        const double* col = LU + i + k0 * ld;
+        const T* col = LU + i + k0 * ld;
         double s = 0.0;
-        for (int j = 0; j < nb; ++j) s += col[j * ld] * xk[j];
+        for (int j = 0; j < nb; ++j) s += (double)col[j * ld] * xk[j];
         r[i] -= s;
     } else {
         // 16 lanes per row, 4 consecutive entries a lane; 64 rows a workgroup
@@ -368,11 +369,11 @@ __global__ __launch_bounds__(256) void k_trsv_step(const double* __restrict__ LU
             const long i = lo + (long)blockIdx.x * 64 + pass * 16 + g;
             double s = 0.0;
             if (i < hi) {
-                const double* row = LU + i * ld + k0;
+                const T* row = LU + i * ld + k0;
 #pragma unroll
                 for (int e = 0; e < 4; ++e) {
                     const int j = 4 * l + e;
-                    if (j < nb) s += row[j] * xk[j];
+                    if (j < nb) s += (double)row[j] * xk[j];
                 }
             }
 #pragma unroll
@@ -394,8 +395,8 @@ __global__ __launch_bounds__(256) void k_permute_vec(const double* __restrict__ 
     else out[perm[i]] = v[i];
 }
 
-template <bool LOWER, bool ROWWISE>
-int trsv_blocked(const double* LU, long n, const double* inv, double* r, double* x, hipStream_t st)
+template <typename T, bool LOWER, bool ROWWISE>
+int trsv_blocked(const T* LU, long n, const double* inv, double* r, double* x, hipStream_t st)
 {
     const long nblk = (n + kTB - 1) / kTB;
     for (long q = 0; q < nblk; ++q) {
@@ -405,7 +406,45 @@ int trsv_blocked(const double* LU, long n, const double* inv, double* r, double*
         const long rows = LOWER ? n - k0 - nb : k0;
         const long per = ROWWISE ? 64 : 256;
         const long g = rows > 0 ? (rows + per - 1) / per : 1;
-        hipLaunchKernelGGL((k_trsv_step<LOWER, ROWWISE>), dim3((unsigned)g), dim3(256), 0, st, LU, n, n, inv, k, r, x);
+        hipLaunchKernelGGL((k_trsv_step<T, LOWER, ROWWISE>), dim3((unsigned)g), dim3(256), 0, st, LU, n, n, inv, k, r, x);
+    }
+    return hipGetLastError() == hipSuccess ? MBEM_OK : MBEM_EHIP;
+}
+
+template <typename T>
+int lu_prepare_impl(const T* LU, int64_t n, double* inv_l, double* inv_u, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!LU || !inv_l || !inv_u))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    const long nblk = (n + kTB - 1) / kTB;
+    hipLaunchKernelGGL((k_tri_inv_diag<T, true>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_l);
+    hipLaunchKernelGGL((k_tri_inv_diag<T, false>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_u);
+    return hipGetLastError() == hipSuccess ? MBEM_OK : MBEM_EHIP;
+}
+
+template <typename T>
+int lu_solve_blocked_impl(const T* LU, int64_t n, const int64_t* perm, const double* inv_l,
+                          const double* inv_u, const double* b, double* x, double* work,
+                          int adjoint, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!LU || !perm || !inv_l || !inv_u || !b || !x || !work))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    double* r = work;
+    double* y = work + n;
+    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    int rc;
+    if (!adjoint) {
+        // U^T w = b, L^T z = w, x = P z
+        if (hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream) != hipSuccess)
+            return MBEM_EHIP;
+        if ((rc = trsv_blocked<T, true, true>(LU, n, inv_u, r, y, stream)) != MBEM_OK) return rc;
+        if ((rc = trsv_blocked<T, false, true>(LU, n, inv_l, y, r, stream)) != MBEM_OK) return rc;
+        hipLaunchKernelGGL((k_permute_vec<false>), grid, block, 0, stream, r, perm, n, x);
+    } else {
+        // z = P^T b, L y = z, U x = y
+        hipLaunchKernelGGL((k_permute_vec<true>), grid, block, 0, stream, b, perm, n, r);
+        if ((rc = trsv_blocked<T, true, false>(LU, n, inv_l, r, y, stream)) != MBEM_OK) return rc;
+        if ((rc = trsv_blocked<T, false, false>(LU, n, inv_u, y, x, stream)) != MBEM_OK) return rc;
     }
     return hipGetLastError() == hipSuccess ? MBEM_OK : MBEM_EHIP;
 }
@@ -604,6 +643,16 @@ int mbem_lu_factor(double* A, int64_t n, int64_t* ipiv, int64_t* info, hipStream
     return launch_status();
 }
 
+int mbem_lu_factor_f32(float* A, int64_t n, int64_t* ipiv, int64_t* info, hipStream_t stream)
+{
+    if (n < 0 || (n > 0 && (!A || !ipiv || !info))) return MBEM_EINVAL;
+    if (n == 0) return MBEM_OK;
+    if (!g_handle && rocblas_create_handle(&g_handle) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_handle, stream) != rocblas_status_success) return MBEM_EHIP;
+    if (rocsolver_sgetrf_64(g_handle, n, n, A, n, ipiv, info) != rocblas_status_success) return MBEM_EHIP;
+    return launch_status();
+}
+
 int mbem_lu_solve(const double* LU, int64_t n, const int64_t* ipiv, double* b, int64_t nrhs,
                   hipStream_t stream)
 {
@@ -649,38 +698,30 @@ int64_t mbem_lu_inv_doubles(int64_t n) { return n <= 0 ? 0 : ((n + kTB - 1) / kT
 
 int mbem_lu_prepare(const double* LU, int64_t n, double* inv_l, double* inv_u, hipStream_t stream)
 {
-    if (n < 0 || (n > 0 && (!LU || !inv_l || !inv_u))) return MBEM_EINVAL;
-    if (n == 0) return MBEM_OK;
-    const long nblk = (n + kTB - 1) / kTB;
-    hipLaunchKernelGGL((k_tri_inv_diag<true>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_l);
-    hipLaunchKernelGGL((k_tri_inv_diag<false>), dim3((unsigned)nblk), dim3(kTB), 0, stream, LU, n, n, inv_u);
-    return launch_status();
+    const int rc = lu_prepare_impl(LU, n, inv_l, inv_u, stream);
+    return rc != MBEM_OK ? rc : launch_status();
+}
+
+int mbem_lu_prepare_f32(const float* LU, int64_t n, double* inv_l, double* inv_u, hipStream_t stream)
+{
+    const int rc = lu_prepare_impl(LU, n, inv_l, inv_u, stream);
+    return rc != MBEM_OK ? rc : launch_status();
 }
 
 int mbem_lu_solve_blocked(const double* LU, int64_t n, const int64_t* perm, const double* inv_l,
                           const double* inv_u, const double* b, double* x, double* work,
                           int adjoint, hipStream_t stream)
 {
-    if (n < 0 || (n > 0 && (!LU || !perm || !inv_l || !inv_u || !b || !x || !work))) return MBEM_EINVAL;
-    if (n == 0) return MBEM_OK;
-    double* r = work;
-    double* y = work + n;
-    const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-    int rc;
-    if (!adjoint) {
-        // U^T w = b, L^T z = w, x = P z
-        if (hipMemcpyAsync(r, b, n * sizeof(double), hipMemcpyDeviceToDevice, stream) != hipSuccess)
-            return MBEM_EHIP;
-        if ((rc = trsv_blocked<true, true>(LU, n, inv_u, r, y, stream)) != MBEM_OK) return rc;
-        if ((rc = trsv_blocked<false, true>(LU, n, inv_l, y, r, stream)) != MBEM_OK) return rc;
-        hipLaunchKernelGGL((k_permute_vec<false>), grid, block, 0, stream, r, perm, n, x);
-    } else {
-        // z = P^T b, L y = z, U x = y
-        hipLaunchKernelGGL((k_permute_vec<true>), grid, block, 0, stream, b, perm, n, r);
-        if ((rc = trsv_blocked<true, false>(LU, n, inv_l, r, y, stream)) != MBEM_OK) return rc;
-        if ((rc = trsv_blocked<false, false>(LU, n, inv_u, y, x, stream)) != MBEM_OK) return rc;
-    }
-    return launch_status();
+    const int rc = lu_solve_blocked_impl(LU, n, perm, inv_l, inv_u, b, x, work, adjoint, stream);
+    return rc != MBEM_OK ? rc : launch_status();
+}
+
+int mbem_lu_solve_blocked_f32(const float* LU, int64_t n, const int64_t* perm, const double* inv_l,
+                              const double* inv_u, const double* b, double* x, double* work,
+                              int adjoint, hipStream_t stream)
+{
+    const int rc = lu_solve_blocked_impl(LU, n, perm, inv_l, inv_u, b, x, work, adjoint, stream);
+    return rc != MBEM_OK ? rc : launch_status();
 }
 
 int mbem_bj_apply(const int64_t* wave_table, int64_t nwaves, const int64_t* chunk_ptr,

@@ -27,6 +27,9 @@ COLLOCATION_SHRINK_BY_ORDER = {0: 0.0, 1: 0.5, 2: 0.5}  # defaults.py:957
 COLLOCATION_FAULT_CLEARANCE_EPS = 1.0  # defaults.py:963
 ASSEMBLY_SOURCE_CHUNK = 256        # defaults.py:981
 DENSE_LU = "double"                # defaults.py:1243
+LU_REFINE_TOL = 1.0e-10            # defaults.py:1250
+LU_REFINE_PASSES = 4               # defaults.py:1255
+LU_REFINE_RATE = 1.0e-2            # defaults.py:1256
 LAPLACE_POINTS_PER_DECADE = 3      # defaults.py:1035
 LAPLACE_PAD_DECADES = 0.5          # defaults.py:1040
 PRONY_RCOND = 1.0e-13              # defaults.py:1067

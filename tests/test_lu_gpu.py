@@ -87,3 +87,90 @@ def test_pivot_order_and_block_inverses():
         assert np.allclose(inv_l[k][:nb, :nb] @ L[k0:k1, k0:k1], np.eye(nb), atol=1e-12)
         assert np.allclose(inv_u[k][:nb, :nb] @ U[k0:k1, k0:k1], np.eye(nb), atol=1e-10)
         assert np.array_equal(inv_l[k][nb:, nb:], np.eye(64 - nb))
+
+
+def _factor32(A):
+    from mbem_amd._lib import call, ptr, stream
+    from mbem_amd.backends.dense import _LUFactors
+    n = A.shape[0]
+    dev = torch.device("cuda")
+    LU = torch.as_tensor(A, device=dev).to(torch.float32).contiguous()
+    ipiv = torch.empty(n, dtype=torch.int64, device=dev)
+    info = torch.zeros(1, dtype=torch.int64, device=dev)
+    call("mbem_lu_factor_f32", ptr(LU), n, ptr(ipiv), ptr(info), stream())
+    assert int(info.item()) == 0
+    return _LUFactors(LU, ipiv, n)
+
+
+@pytest.mark.parametrize("n", [65, 300, 1000])
+def test_single_precision_factors_and_refinement(n):
+    """FP32 factors (sgetrf + the blocked solves reading FP32 factors with
+    FP64 vectors): one solve is within single precision times the condition
+    number of numpy's FP64 solve, both directions; refined against the FP64
+    matrix (la/lu.py:198-239) it stops within 3 corrections with the
+    predicted remainder below 1e-10, and the actual error is below that."""
+    from mbem_amd.backends.dense import refine
+    from mbem_amd.la.solver import gemv
+    rng = np.random.default_rng(n + 40)
+    A = rng.standard_normal((n, n)) + 0.5 * np.sqrt(n) * np.eye(n)
+    b = rng.standard_normal(n)
+    fac = _factor32(A)
+    dev = torch.device("cuda")
+    bd = torch.as_tensor(b, device=dev)
+    ref = np.linalg.solve(A, b)
+    kappa = np.linalg.cond(A, 1)
+    x = torch.empty_like(bd)
+    fac.solve_into(bd, x)
+    e1 = np.linalg.norm(x.cpu().numpy() - ref) / np.linalg.norm(ref)
+    fac.solve_into(bd, x, adjoint=True)
+    et = np.linalg.norm(x.cpu().numpy() - np.linalg.solve(A.T, b)) / np.linalg.norm(ref)
+    Ad = torch.as_tensor(A, device=dev)
+    xr, rep = refine(fac.solve_into, lambda v, out: gemv(Ad, v, out), bd)
+    er = np.linalg.norm(xr.cpu().numpy() - ref) / np.linalg.norm(ref)
+    print(f"n {n}: cond {kappa:.1e}; FP32 factors {e1:.1e} (A^T {et:.1e}); refined {er:.1e} "
+          f"after {rep['passes']} corrections (left {rep['left']:.1e})", flush=True)
+    assert e1 < 1e-6 * kappa and et < 1e-5 * kappa
+    assert rep["done"] and rep["passes"] <= 3 and rep["left"] <= 1e-10 and er <= 1e-10
+
+
+@pytest.mark.parametrize("how", ["single", "gpu"])
+def test_dense_lu_single_precision_matches_double(how):
+    """lu="single" / "gpu" on the assembled C1-size box: the refined solution
+    within 1e-10 of the FP64 LU's, the report carrying the refinement, the
+    condition estimate (the same estimator on the FP32 factors) within a
+    factor 2 of gecon's on the FP64 ones.  And the fallback: a refinement that
+    does not contract (forced) warns and re-factors in double."""
+    import warnings
+
+    import mbem_amd.backends.dense as dense
+    from mbem_amd.backends.dense import AssembledDense
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.model import generate_system
+    model, eps, _ = box_fault_model(16)
+    system = generate_system(model)
+    ref = AssembledDense(system, eps)
+    x64 = ref.solve(method="lu", return_device=True)
+    c64 = ref.report.cond_estimate
+    asm = AssembledDense(system, eps, lu=how)
+    x = asm.solve(method="lu", return_device=True)
+    rep = asm.report
+    e = float(torch.linalg.vector_norm(x - x64) / torch.linalg.vector_norm(x64))
+    print(f"{how}: {rep.lu}; vs double LU {e:.1e}; cond {rep.cond_estimate:.2e} vs {c64:.2e}",
+          flush=True)
+    assert rep.lu["how"] == how and 1 <= rep.lu["passes"] <= 4 and rep.lu["left"] <= 1e-10
+    assert e < 1e-10 and rep.residual < 1e-12
+    assert 0.5 <= rep.cond_estimate / c64 <= 2.0
+    real = dense.refine
+    try:
+        dense.refine = lambda *a, **k: (real(*a, **k)[0], {"done": False, "passes": 4,
+                                                            "correction": 1.0, "left": 1.0,
+                                                            "rate": 0.5})
+        asm.release()
+        with warnings.catch_warnings(record=True) as w:
+            warnings.simplefilter("always")
+            x2 = asm.solve(method="lu", return_device=True)
+        assert any("did not contract" in str(m.message) for m in w)
+    finally:
+        dense.refine = real
+    e2 = float(torch.linalg.vector_norm(x2 - x64) / torch.linalg.vector_norm(x64))
+    assert asm.report.lu.get("fallback") is True and e2 < 1e-12
