@@ -218,3 +218,4 @@ def test_warn_window():
     with warnings.catch_warnings():
         warnings.simplefilter("error")
         assert rh.warn_window({"a": ok, "b": band_above}, s) == []
+
