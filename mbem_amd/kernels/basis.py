@@ -105,6 +105,103 @@ def _field_points(mesh_or_points):
     return np.ascontiguousarray(np.asarray(mesh_or_points, dtype=float))
 
 
+class MeshArrays:
+    """Memo of mesh-derived arrays for one assembly or readout (basis.py:91-150):
+    source vertices and normals, field points, collocation points and normals,
+    each computed once per object.  Keyed by ``id()``, so it must not outlive
+    the build it serves (a mesh can be mutated in place, an id reused)."""
+
+    def __init__(self):
+        self._memo: dict = {}
+
+    def _get(self, key, make):
+        v = self._memo.get(key)
+        if v is None:
+            v = make()
+            self._memo[key] = v
+        return v
+
+    def source_arrays(self, mesh_source):
+        return self._get(("src", id(mesh_source)), lambda: _source_arrays(mesh_source))
+
+    def field_points(self, mesh_or_points):
+        if not hasattr(mesh_or_points, "centroids"):
+            return _field_points(mesh_or_points)
+        return self._get(("pts", id(mesh_or_points)), lambda: _field_points(mesh_or_points))
+
+    def collocation_points(self, patch):
+        return self._get(("coll", id(patch)),
+                         lambda: np.ascontiguousarray(patch.collocation_points()))
+
+    def collocation_normals(self, patch):
+        return self._get(("nrm", id(patch)),
+                         lambda: np.ascontiguousarray(patch.collocation_normals()))
+
+
+# Lagrange nodes of a triangle in the nodal kernels' order (basis.py:152-185):
+# order 0 the centroid; 1 the vertices; 2 the vertices, then the midpoints of
+# edges 12, 23, 31.  Node k of element s is density row K s + k.
+
+def lagrange_nodes(order: int, shrink: float = 0.0) -> np.ndarray:
+    """Barycentric coordinates (K, 3) of the nodes, each pulled toward the
+    centroid by ``shrink`` t: (1 - t) lam + t / 3 (a collocation point)."""
+    from .tri_nodal import n_nodes
+    K = n_nodes(order)
+    if order == 0:
+        lam = np.full((1, 3), 1.0 / 3.0)
+    else:
+        lam = np.eye(3)
+        if order == 2:
+            lam = np.vstack([lam, 0.5 * (lam + np.roll(lam, -1, axis=0))])
+    t = float(shrink)
+    if t:
+        lam = (1.0 - t) * lam + t / 3.0
+    assert lam.shape == (K, 3)
+    return lam
+
+
+def lagrange_shape(order: int, lam) -> np.ndarray:
+    """Shape functions (M, K) at barycentric points ``lam`` (M, 3): 1; the
+    barycentrics; l_i (2 l_i - 1) at the vertices and 4 l_i l_j at the edge
+    midpoints."""
+    from .tri_nodal import n_nodes
+    lam = np.atleast_2d(np.asarray(lam, float))
+    if n_nodes(order) == 1:
+        return np.ones((lam.shape[0], 1))
+    if order == 1:
+        return lam.copy()
+    nxt = np.roll(lam, -1, axis=1)
+    return np.hstack([lam * (2.0 * lam - 1.0), 4.0 * lam * nxt])
+
+
+def identity_ids(order: int, n_source: int) -> np.ndarray:
+    """The element-major layout as an ids table (N_s, K): ids[s, k] = K s + k."""
+    from .tri_nodal import n_nodes
+    K = n_nodes(order)
+    return np.arange(K * int(n_source), dtype=np.int64).reshape(int(n_source), K)
+
+
+def condense_columns(blk, ids, n_cols: int, out=None) -> np.ndarray:
+    """Element-major columns (R, 3 K N_s) summed into shared node columns
+    (R, 3 n_cols): column 3 (K s + k) + j is ADDED to column 3 ids[s, k] + j
+    (basis.py:283-301); ``out`` accumulates in place when given."""
+    blk = np.asarray(blk, float)
+    ids = np.asarray(ids, dtype=np.int64)
+    N_s, K = ids.shape
+    if blk.ndim != 2 or blk.shape[1] != 3 * K * N_s:
+        raise ValueError(f"block has {blk.shape[-1]} columns, ids imply {3 * K * N_s}")
+    shape = (blk.shape[0], 3 * int(n_cols))
+    if out is None:
+        out = np.zeros(shape)
+    elif out.shape != shape:
+        raise ValueError(f"out shape {out.shape} != {shape}")
+    if ids.size and (ids.min() < 0 or ids.max() >= n_cols):
+        raise ValueError(f"ids must index 0 .. n_cols - 1 = {int(n_cols) - 1}")
+    dst = (3 * ids.reshape(-1, 1) + np.arange(3)).ravel()
+    np.add.at(out.T, dst, blk.T)
+    return out
+
+
 @dataclass
 class UBasis:
     """Geometry-only U basis stack (3, 3N_f, 3N_s) (basis.py:304-314)."""
@@ -181,3 +278,12 @@ def assemble_t_matrix(mesh_field, mesh_source, material, eps, arrays=None, order
     c = t_coeffs(material.mu, material.lam)
     return tk.t_matrix_direct(x_field, tri_verts, normals,
                               as_eps_array(eps, tri_verts.shape[0]), *c)
+
+
+def assemble_traction_matrix(x_field, n_field, mesh_source, material, eps, arrays=None,
+                             order: int = 0, kind: str = "slip", ids=None, n_cols=None):
+    """Traction rows of a nodal density (basis.py:406-430): the rows of solved
+    (FAULT_TRACTION) faults, not in this build."""
+    raise NotImplementedError("assemble_traction_matrix: the traction rows of solved "
+                              "(FAULT_TRACTION) faults are not in this build")
+
