@@ -22,6 +22,7 @@ Bars (FP64, north star 1e-10 relative):
 """
 import numpy as np
 import pytest
+import torch
 
 pytestmark = pytest.mark.gpu
 
@@ -178,6 +179,34 @@ def test_fault_traction(c1):
     assert np.array_equal(evaluate_fault_traction(model, region.name, sol, fault.name, eps), t)
     with pytest.raises(KeyError):
         evaluate_fault_traction(model, region, sol, "no such fault", eps)
+
+
+def test_assembly_modes(c1):
+    """dense.py:1-30, 49-86, 650-687: "basis" and "legacy" assemble the same
+    operator as "direct" (here through the same device kernels, so bit for
+    bit); legacy keeps the reference's restrictions -- one global scalar eps,
+    no material rebuilds -- and an unknown mode or LU flavour is refused."""
+    from mbem_amd.backends.dense import DenseBackend
+    from mbem_amd.mesh import ElasticMaterial
+    system, eps = c1["system"], c1["eps"]
+    direct = DenseBackend().assemble(system, eps)
+    basis = DenseBackend(mode="basis").assemble(system, eps)
+    assert torch.equal(basis.A_dev, direct.A_dev) and torch.equal(basis.b_dev, direct.b_dev)
+    region = c1["model"].regions[0].name
+    soft = ElasticMaterial(mu=20.0, lam=30.0)
+    assert torch.equal(basis.rebuild_for_materials({region: soft}).A_dev,
+                       direct.rebuild_for_materials({region: soft}).A_dev)
+    with pytest.raises(ValueError):
+        DenseBackend(mode="legacy").assemble(system, eps if isinstance(eps, dict) else {"x": eps})
+    h = 0.5
+    legacy = DenseBackend(mode="legacy").assemble(system, h)
+    assert torch.equal(legacy.A_dev, DenseBackend().assemble(system, h).A_dev)
+    with pytest.raises(ValueError):
+        legacy.rebuild_for_materials({region: soft})
+    with pytest.raises(ValueError):
+        DenseBackend(mode="fast")
+    with pytest.raises(ValueError):
+        DenseBackend(lu="half")
 
 
 def test_rigid_translation_gives_zero_traction():
