@@ -538,7 +538,7 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=1500)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
-    ap.add_argument("--recycle", type=int, default=60,
+    ap.add_argument("--recycle", type=int, default=90,
                     help="recycled vectors kept across the mixed method's inner solves")
     ap.add_argument("--workload", choices=("solve", "laplace", "readout", "lu"), default="solve",
                     help="solve: the N=20k dense solve (the headline metric); laplace: the "

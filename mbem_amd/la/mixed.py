@@ -360,7 +360,7 @@ class MixedReport:
 def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
                 inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
                 outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=1500,
-                precond_grouping: str = "region", pattern=None, recycle: int = 60):
+                precond_grouping: str = "region", pattern=None, recycle: int = 90):
     """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport).
     ``recycle`` > 0 keeps up to that many recycled vectors across the inner
     solves (``_RecycledInner``)."""
