@@ -44,8 +44,9 @@ extern "C" {
 #define MBEM_BJ32_ROWS_PER_WAVE 8
 
 /* rows of the int8 far field one wavefront multiplies (mbem_gemv_i8), and
- * the 16-byte loads per row each lane keeps in flight (measured at n = 60,000:
- * 8 x 2 623 us, 8 x 3 625 us, 4 x 4 644 us, 4 x 3 652 us) */
+ * the 16-byte loads per row each lane keeps in flight (measured at n = 60,000
+ * before x was staged through LDS: 8 x 2 623 us, 8 x 3 625 us, 4 x 4 644 us,
+ * 4 x 3 652 us; with LDS staging 8 x 2 593 us) */
 #define MBEM_I8_ROWS_PER_WAVE 8
 #define MBEM_I8_LOADS_PER_ROW 2
 

@@ -70,11 +70,15 @@ def test_mixed_solve(c1, storage, recycle):
     assert err < 1e-9
 
 
-@pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030)])
+@pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030), (45, 5000),
+                                 (3, 8208)])
 def test_int8_cast_and_product_match_oracle(m, n):
     """Random rows spanning 25 decades, a row 1e-11 smaller, an all-zero
     segment, and column counts that leave a partial last segment, a partial
-    last 1024-column block and a tail past the last 16 bytes."""
+    last 1024-column block and a tail past the last 16 bytes.  The product
+    stages x through LDS 2048 columns at a time: 2500 columns use one tile,
+    5000 two (both buffers) with waves past the last row in the second
+    workgroup, 8208 eight tiles and a one-vector remainder."""
     import torch
     from mbem_amd._lib import call, load, ptr, stream
     from oracle import int8 as oi
