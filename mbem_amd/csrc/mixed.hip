@@ -100,7 +100,8 @@ __global__ __launch_bounds__(256) void k_near_extract(const double* __restrict__
 // y = alpha (F x) + beta y with F stored low precision and x in FP32;
 // FP32 products and accumulation, FP64 result.  One wave per R rows; each
 // lane keeps two 16-byte row loads in flight per row (columns c and c + 512
-// for BF16), which measured fastest on gfx950 for n = 60,000 (6.4 TB/s).
+// for BF16), which measured fastest on gfx950 for n = 60,000 (6.4 TB/s; 6.8
+// TB/s, 1.065 ms for BF16, once x was staged through LDS).
 template <int R, bool BF>
 __device__ __forceinline__ void lo_fma8(float& acc, const u32x4 w, const flt4 xa, const flt4 xb)
 {
@@ -113,15 +114,20 @@ __device__ __forceinline__ void lo_fma4(float& acc, const flt4 w, const flt4 xa)
     acc += w.x * xa.x + w.y * xa.y + w.z * xa.z + w.w * xa.w;
 }
 
+// x goes through LDS 128 vectors (1,024 BF16 or 512 FP32 columns) at a time,
+// double-buffered, one barrier per tile, as in k_gemv_i8; each lane keeps the
+// column order of the plain loop (bit-identical sums).  No wave returns
+// early (clamped rows, guarded stores).  Requires 256 threads per block.
 template <int R, bool BF>
 __global__ __launch_bounds__(256) void k_gemv_lo(const void* __restrict__ F, long ldf, long m, long n,
                                                  const float* __restrict__ x,
                                                  double* __restrict__ y, double alpha, double beta)
 {
+    constexpr int kXq = BF ? 256 : 128;  // flt4 of x per tile
+    __shared__ flt4 xs[2][kXq];
     const int lane = threadIdx.x & 63;
     const long wave = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     const long r0 = wave * R;
-    if (r0 >= m) return;
     float acc[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) acc[r] = 0.0f;
@@ -133,7 +139,52 @@ __global__ __launch_bounds__(256) void k_gemv_lo(const void* __restrict__ F, lon
     long rows[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) rows[r] = ((r0 + r < m) ? r0 + r : m - 1) * ldf;
-    long v = lane;
+    const long ntile = nvec / 128;
+    if (ntile > 0) {
+        if (threadIdx.x < kXq) xs[0][threadIdx.x] = reinterpret_cast<const flt4*>(x)[threadIdx.x];
+        __syncthreads();
+    }
+    for (long k = 0; k < ntile; ++k) {
+        const bool more = k + 1 < ntile;
+        flt4 xn;
+        if (more && threadIdx.x < kXq)
+            xn = reinterpret_cast<const flt4*>(x + (k + 1) * 128 * E)[threadIdx.x];
+        const long c = (k * 128 + lane) * E, c2 = c + 64 * E;
+        const flt4* xt = xs[k & 1];
+        if (BF) {
+            u32x4 w[R], w2[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                w[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Fb + rows[r] + c));
+                w2[r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(Fb + rows[r] + c2));
+            }
+            const flt4 xa = xt[2 * lane], xb = xt[2 * lane + 1];
+            const flt4 xc = xt[128 + 2 * lane], xd = xt[128 + 2 * lane + 1];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                lo_fma8<R, BF>(acc[r], w[r], xa, xb);
+                lo_fma8<R, BF>(acc[r], w2[r], xc, xd);
+            }
+        } else {
+            flt4 w[R], w2[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                w[r] = __builtin_nontemporal_load(reinterpret_cast<const flt4*>(Ff + rows[r] + c));
+                w2[r] = __builtin_nontemporal_load(reinterpret_cast<const flt4*>(Ff + rows[r] + c2));
+            }
+            const flt4 xa = xt[lane], xc = xt[64 + lane];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                lo_fma4(acc[r], w[r], xa);
+                lo_fma4(acc[r], w2[r], xc);
+            }
+        }
+        if (more) {
+            if (threadIdx.x < kXq) xs[(k + 1) & 1][threadIdx.x] = xn;
+            __syncthreads();
+        }
+    }
+    long v = ntile * 128 + lane;
     for (; v + 64 < nvec; v += 128) {
         const long c = v * E, c2 = c + 64 * E;
         if (BF) {
