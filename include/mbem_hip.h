@@ -233,7 +233,7 @@ int mbem_bj_apply(const int64_t *wave_table, int64_t nwaves, const int64_t *chun
                   const int64_t *idx, const int64_t *inv_ptr, const double *inv, const double *r,
                   double *z, hipStream_t stream);
 
-/* ---- mixed precision (north-star pattern 1) ---------------------------- */
+/* ---- mixed precision ------------------------------------------------- */
 
 /* out = lo(A) for an (m, n) row-major FP64 block: BF16 bits (bf16 != 0, round
  * to nearest even) or FP32, leading dimension ldo.  Far-field storage of the

@@ -1,4 +1,4 @@
-"""Mixed-precision solve of an assembled dense system (north-star pattern 1).
+"""Mixed-precision solve of an assembled dense system.
 
 Operator split A = F + N on the device:
   N  near-field and self-element 3x3 blocks in FP64 block-CSR -- every

@@ -4,7 +4,7 @@ geometries: points on the element, on edges and vertices, in the source plane
 outside it, far away; per-element eps; needles; nu = 0.25 / 0.30 / 0.45;
 degenerate sources and zero densities.
 
-Tolerance (north star: FP64 results within 1e-10 relative): every entry
+Tolerance (FP64 results within 1e-10 relative): every entry
 within 1e-10 of the largest entry of its matrix, and the Frobenius norm of the
 difference within 1e-12 of the matrix's.
 """

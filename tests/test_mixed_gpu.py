@@ -1,4 +1,4 @@
-"""Mixed-precision operator and solve (north-star pattern 1) on the GPU, on
+"""Mixed-precision operator and solve on the GPU, on
 the 2,048-triangle box with its dipping fault.
 
   * the near-field blocks are copied exactly (FP64) and the low copy is zero

@@ -2,7 +2,7 @@
 oracle, on the CPU-runnable box-with-dipping-fault workload (N = 2,048 box
 triangles, n = 6,144 unknowns, 400 fault triangles, eps = 0.5 h).
 
-Bars (FP64, north star 1e-10 relative):
+Bars (FP64, 1e-10 relative):
   A, b, calibrated diagonal : every entry within 1e-10 of max|.|, Frobenius
                               within 1e-12;
   LU solution               : within 1e-10 (relative 2-norm) of the oracle's LU;
