@@ -17,7 +17,7 @@ The learned preconditioner's A/B on the same system is reported beside it.
 
 ``--workload laplace`` (batched Laplace-state solves) and ``--workload
 readout`` (stress-readout throughput) print their own JSON lines.
-``--gpus`` above 1 exits: this build measures one GPU.
+``--gpus`` above 1 exits: bench.py measures one GPU.
 """
 from __future__ import annotations
 
@@ -563,7 +563,7 @@ def main():
         kernel_probe(args.probe_traffic, 3)
         return
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported "
               f"(the dense path's sharding is described, not run)", file=sys.stderr)
         sys.exit(2)
     if args.workload == "laplace":

@@ -157,7 +157,7 @@ def load():
 def require_gpu():
     if not torch.cuda.is_available():
         raise MbemError("libmbem_hip needs a ROCm GPU (torch.cuda.is_available() is False); "
-                        "this build has no CPU path")
+                        "mbem_amd has no CPU path")
     load()
 
 

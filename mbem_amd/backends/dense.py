@@ -325,7 +325,7 @@ class AssembledDense:
         if mode == "legacy" and isinstance(eps, (dict, str)):
             raise ValueError("legacy mode supports only global scalar eps")
         if fault_drag:
-            raise NotImplementedError("fault drag needs FAULT_TRACTION faults (not in this build)")
+            raise NotImplementedError("fault drag needs FAULT_TRACTION faults (not in mbem_amd)")
         _lib.require_gpu()
         t0 = time.perf_counter()
         require_anchor_or_deflate(system, jump, deflate)
@@ -477,7 +477,7 @@ class AssembledDense:
                 dv.u_block(x, src, c, out, lda, scale=t.scale, beta=beta)
                 n_u += x.shape[0] * src.ns
             else:
-                raise NotImplementedError(f"kernel {t.kernel!r} is not in this build")
+                raise NotImplementedError(f"kernel {t.kernel!r} is not in mbem_amd")
         # zero the blocks no term writes
         slots = self.layout.slots
         for rs_ in slots:

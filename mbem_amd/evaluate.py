@@ -98,7 +98,7 @@ def evaluate_displacement(model: RegionModel, region, solution: dict, points, ep
                           warn_near: bool = True, near=None) -> np.ndarray:
     """Displacement (N,3) at interior ``points`` of ``region`` (evaluate.py:194-253)."""
     if near is not None:
-        raise NotImplementedError("near-face recovery (near=) is not in this build")
+        raise NotImplementedError("near-face recovery (near=) is not in mbem_amd")
     region = _region(model, region)
     if region.faults:
         from .selfcheck import ensure_fault_convention
@@ -202,7 +202,7 @@ def evaluate_stress(model: RegionModel, region, solution: dict, points, eps,
     """Stress (N,3,3) at ``points`` of ``region`` (evaluate.py:598-695);
     ``parts`` also returns the eigenstress removed."""
     if near is not None:
-        raise NotImplementedError("near-face recovery (near=) is not in this build")
+        raise NotImplementedError("near-face recovery (near=) is not in mbem_amd")
     region = _region(model, region)
     if region.faults:
         from .selfcheck import ensure_fault_convention

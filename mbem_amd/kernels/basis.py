@@ -258,7 +258,7 @@ def assemble_u_matrix(mesh_field, mesh_source, material, eps, arrays=None, order
     from . import tri_kernels as tk
     _order0(order)
     if ids is not None:
-        raise NotImplementedError("shared-node columns (continuous layouts) are not in this build")
+        raise NotImplementedError("shared-node columns (continuous layouts) are not in mbem_amd")
     x_field = _field_points(mesh_field)
     tri_verts, _ = _source_arrays(mesh_source)
     g1, g2, g3 = u_coeffs(material.mu, material.lam)
@@ -272,7 +272,7 @@ def assemble_t_matrix(mesh_field, mesh_source, material, eps, arrays=None, order
     from . import tri_kernels as tk
     _order0(order)
     if ids is not None:
-        raise NotImplementedError("shared-node columns (continuous layouts) are not in this build")
+        raise NotImplementedError("shared-node columns (continuous layouts) are not in mbem_amd")
     x_field = _field_points(mesh_field)
     tri_verts, normals = _source_arrays(mesh_source)
     c = t_coeffs(material.mu, material.lam)
@@ -283,7 +283,7 @@ def assemble_t_matrix(mesh_field, mesh_source, material, eps, arrays=None, order
 def assemble_traction_matrix(x_field, n_field, mesh_source, material, eps, arrays=None,
                              order: int = 0, kind: str = "slip", ids=None, n_cols=None):
     """Traction rows of a nodal density (basis.py:406-430): the rows of solved
-    (FAULT_TRACTION) faults, not in this build."""
+    (FAULT_TRACTION) faults, not in mbem_amd."""
     raise NotImplementedError("assemble_traction_matrix: the traction rows of solved "
-                              "(FAULT_TRACTION) faults are not in this build")
+                              "(FAULT_TRACTION) faults are not in mbem_amd")
 

@@ -4,7 +4,7 @@ centroid) is what every configuration reaches, and the reference routes it to
 tri_kernels' pair code (tri_nodal.py:1259-1355); here it routes to this
 package's device kernels (``kernels.tri_kernels``) with the same arguments.
 Lagrange P1 / P2 densities (order 1 / 2) and the traction rows of solved
-faults (``traction_matrix``, FAULT_TRACTION) are not in this build and raise
+faults (``traction_matrix``, FAULT_TRACTION) are not in mbem_amd and raise
 NotImplementedError; a malformed order raises ValueError as the reference's
 ``n_nodes`` does.
 """
@@ -24,7 +24,7 @@ def n_nodes(order: int) -> int:
 def _order0(order: int, name: str) -> None:
     if n_nodes(order) != 1:
         raise NotImplementedError(f"{name}: order {order} (Lagrange P{order} nodal densities) "
-                                  f"is not in this build; order 0 is")
+                                  f"is not in mbem_amd; order 0 is")
 
 
 def u_basis_matrices(x_field, tri_verts, eps_arr, order):
@@ -87,9 +87,9 @@ def eigenstress_contract(x_field, tri_verts, normals, eps_arr, density, mu, lam,
 def traction_matrix(x_field, n_field, tri_verts, normals, eps_arr, ids, n_cols, mu, lam, order,
                     which="slip"):
     """Traction rows of a nodal density (tri_nodal.py:1370-1404): the rows of
-    solved (FAULT_TRACTION) faults, not in this build at any order."""
+    solved (FAULT_TRACTION) faults, not in mbem_amd at any order."""
     n_nodes(order)
     if which not in ("slip", "force", "eigen", "total"):
         raise KeyError(which)
     raise NotImplementedError("traction_matrix: the traction rows of solved (FAULT_TRACTION) "
-                              "faults are not in this build")
+                              "faults are not in mbem_amd")

@@ -151,7 +151,7 @@ class BlockGaussSeidel:
             bj_chunk = defaults.PRECOND_BJ_CHUNK_DOF
         if above_dense != "block_jacobi":
             raise NotImplementedError(f"rung {above_dense!r} above the dense cap: only "
-                                      f"'block_jacobi' is in this build")
+                                      f"'block_jacobi' is in mbem_amd")
         if grouping == "region" and getattr(assembled, "deflate", False):
             grouping = "patch"
         if inverse_dtype not in ("f64", "f32"):

@@ -9,7 +9,7 @@ point (core.py:713-729) and validated by the Gauss closure identity
 ``FAULT_ORIENTATION = -1`` (core.py:92-100): their slip enters every site
 through the same ``-sigma`` coefficient a prescribed displacement does.
 
-Not in this build (each raises NotImplementedError where it is asked for):
+Not in mbem_amd (each raises NotImplementedError where it is asked for):
 Lagrange orders 1 and 2, continuous layouts, FAULT_TRACTION faults,
 interfaces that slip, gravity and far-field windows.
 """
@@ -66,7 +66,7 @@ class Patch:
             raise ValueError(f"patch '{self.name}': order must be 0, 1 or 2")
         if self.order != 0:
             raise NotImplementedError(f"patch '{self.name}': Lagrange order {self.order} is not "
-                                      f"in this build (P0 only)")
+                                      f"in mbem_amd (P0 only)")
         if self.layout not in ("discontinuous", "continuous"):
             raise ValueError(f"patch '{self.name}': layout must be 'discontinuous' or "
                              f"'continuous' (got {self.layout!r})")
@@ -74,7 +74,7 @@ class Patch:
             raise ValueError(f"patch '{self.name}': a continuous layout is P2 only")
         if self.bc is BCType.FAULT_TRACTION:
             raise NotImplementedError(f"patch '{self.name}': FAULT_TRACTION faults (solved slip, "
-                                      f"traction rows) are not in this build")
+                                      f"traction rows) are not in mbem_amd")
         if self.allow_opening or self.locked is not None:
             raise ValueError(f"patch '{self.name}': allow_opening and locked apply to a "
                              f"FAULT_TRACTION fault only")
@@ -84,7 +84,7 @@ class Patch:
             if self.bc is not BCType.INTERFACE:
                 raise ValueError(f"patch '{self.name}': slip is of an INTERFACE patch")
             raise NotImplementedError(f"patch '{self.name}': interfaces that slip are not in "
-                                      f"this build")
+                                      f"mbem_amd")
         self._refuse_degenerate()
 
     def _refuse_degenerate(self) -> None:
@@ -201,9 +201,9 @@ class RegionModel:
 
     def __init__(self, regions, gravity=None, far_field=None):
         if gravity is not None:
-            raise NotImplementedError("gravity (buoyancy terms) is not in this build")
+            raise NotImplementedError("gravity (buoyancy terms) is not in mbem_amd")
         if far_field is not None:
-            raise NotImplementedError("far-field windows are not in this build")
+            raise NotImplementedError("far-field windows are not in mbem_amd")
         self.regions = list(regions)
         self.gravity = None
         self.far_field = None

@@ -11,7 +11,7 @@ with H the T kernel, G the U kernel, prescribed columns moved to the right-
 hand side with their left-hand coefficients, and D_q the collocation free
 term: COLLOCATION_JUMP * I (jump="half") or the calibrated C_q (jump=
 "calibrated").  The fault-convention self-check the reference runs here
-(selfcheck.ensure_fault_convention) runs in this build where the kernels
+(selfcheck.ensure_fault_convention) runs in mbem_amd where the kernels
 run: the GPU dense backend and the evaluators call it.
 """
 from __future__ import annotations
@@ -93,7 +93,7 @@ def generate_system(model: RegionModel) -> BlockSystem:
         for f in region.faults:
             if f.bc is not BCType.FAULT:
                 raise NotImplementedError(f"fault '{f.name}': only prescribed-slip FAULTs are "
-                                          f"in this build")
+                                          f"in mbem_amd")
         for q in region.patches:
             row = layout.row_slot(region, q)
             for p in region.patches:

@@ -65,7 +65,7 @@ def split_coeffs(kernel: str, part: int) -> np.ndarray:
         return T_SPLIT[part]
     if kernel == KERNEL_U:
         return U_SPLIT[part]
-    raise NotImplementedError(f"kernel {kernel!r} has no Laplace split in this build")
+    raise NotImplementedError(f"kernel {kernel!r} has no Laplace split in mbem_amd")
 
 
 @dataclass
