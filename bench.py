@@ -22,6 +22,7 @@ readout`` (stress-readout throughput) print their own JSON lines.
 from __future__ import annotations
 
 import argparse
+import gc
 import json
 import os
 import sys
@@ -66,6 +67,45 @@ def i8_bytes(n: int) -> int:
     """Algorithmic bytes of one int8 far-field matvec (n x n): the codes, one
     FP32 scale per row and 64 columns, x in FP32, y in FP64."""
     return n * n + 4 * n * (-(-n // 64)) + 4 * n + 8 * n
+
+
+def _release():
+    """Free what the last step left: collect garbage first (a dropped
+    object still in a reference cycle keeps its device memory until then),
+    then return the cached blocks to the device."""
+    import torch
+    gc.collect()
+    torch.cuda.empty_cache()
+
+
+def quantity_errors(model, system, eps, x, x_ref):
+    """Relative errors of a solution ``x`` against the FP64 reference
+    ``x_ref`` (both device vectors over the system's unknowns), per quantity:
+    the u-slots (displacement), the t-slots (traction), and the subtracted
+    stress read on the 50 x 50 grid over the top face (z = 0) and 4 eps of
+    the top below it.  Each is max |difference| / max |reference|."""
+    import torch
+    from mbem_amd.cases import surface_grid
+    from mbem_amd.evaluate import evaluate_stress
+    layout = system.layout
+    xh, rh = x.cpu().numpy(), x_ref.cpu().numpy()
+    out = {}
+    for kind in ("u", "t"):
+        idx = np.concatenate([np.arange(s_.offset, s_.stop) for s_ in layout.slots
+                              if s_.kind == kind] or [np.zeros(0, int)])
+        if idx.size:
+            out[kind] = float(np.abs(xh[idx] - rh[idx]).max() / np.abs(rh[idx]).max())
+    region = model.regions[0]
+    sol = {s_.name: xh[s_.offset:s_.stop].reshape(-1, 3) for s_ in layout.slots}
+    ref = {s_.name: rh[s_.offset:s_.stop].reshape(-1, 3) for s_ in layout.slots}
+    top_eps = float(eps["top"]) if isinstance(eps, dict) else float(eps)
+    for tag, z in (("stress_z0", 0.0), ("stress_4eps", -4.0 * top_eps)):
+        pts = surface_grid(50, half=20.0, z=z)
+        sa = evaluate_stress(model, region, sol, pts, eps, warn_near=False)
+        sb = evaluate_stress(model, region, ref, pts, eps, warn_near=False)
+        out[tag] = float(np.abs(sa - sb).max() / np.abs(sb).max())
+    torch.cuda.synchronize()
+    return out
 
 
 def _events_time(fn, reps):
@@ -233,7 +273,7 @@ def bench_laplace(args):
     x = rep = ls = None
     for k in range(args.warmup + args.steps):
         ls = None
-        torch.cuda.empty_cache()
+        _release()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         ls = LaplaceStates(system, eps, {"crust": rh.Maxwell(tau)}, bj_chunk=args.bj_chunk)
@@ -284,7 +324,7 @@ def bench_laplace(args):
             ref = xs[r].cpu().numpy()
             errs.append(float(np.linalg.norm(x[i, r] - ref) / np.linalg.norm(ref)))
         del asm, M
-        torch.cuda.empty_cache()
+        _release()
         print(f"[laplace] sequential state {st.label}: {seq[-1]:.3f} s, "
               f"max rel diff vs batched {max(errs[-len(slips):]):.2e}", flush=True)
     seq_est = float(np.mean(seq)) * len(states)
@@ -444,7 +484,7 @@ def learned_ab(system, eps, args):
     print(f"[bench] learned A/B: {out}", flush=True)
     asm.release()
     asm.A_dev = None
-    torch.cuda.empty_cache()
+    _release()
     return out
 
 
@@ -538,6 +578,9 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=1500)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
+    ap.add_argument("--other-storage-steps", type=int, default=3,
+                    help="steps of the mixed method with the other far-field storage (BF16 "
+                         "beside int8), reported beside the headline")
     ap.add_argument("--recycle", type=int, default=90,
                     help="recycled vectors kept across the mixed method's inner solves")
     ap.add_argument("--workload", choices=("solve", "laplace", "readout", "lu"), default="solve",
@@ -597,7 +640,7 @@ def main():
     asm = None
     for w in range(args.warmup):
         asm = None
-        torch.cuda.empty_cache()
+        _release()
         asm, _ = step()
         print(f"[bench] warmup {w}: assembly {asm.assembly.seconds:.3f} s, {asm.report}",
               flush=True)
@@ -638,17 +681,17 @@ def main():
               f"{ref_bytes / ref_ms / 1e6:.0f} GB/s", flush=True)
         del op
     asm = None
-    torch.cuda.empty_cache()
+    _release()
     print(f"[bench] gemv {gemv_ms:.3f} ms = {gemv_bytes / gemv_ms / 1e6:.0f} GB/s setup "
           f"{time.perf_counter() - t_setup:.1f} s", flush=True)
 
-    def timed(step_fn, label):
+    def timed(step_fn, label, steps):
         times, errs, iters = [], [], []
-        asm_k = None
+        asm_k = x = rep = None
         torch.cuda.synchronize()
-        for k in range(args.steps):
-            asm_k = None
-            torch.cuda.empty_cache()
+        for k in range(steps):
+            asm_k = x = None
+            _release()
             ts = time.perf_counter()
             asm_k, x, rep = step_fn()
             torch.cuda.synchronize()
@@ -659,17 +702,19 @@ def main():
             ph = ", ".join(f"{k_}={v:.3f}" for k_, v in asm_k.timings.items())
             print(f"[bench] {label} step {k}: {times[-1]:.3f} s ({ph}; {rep}; "
                   f"rel err {errs[-1]:.2e})", flush=True)
-        return times, errs, iters, asm_k, rep
+        summary = getattr(rep, "precond_summary", None)
+        del asm_k
+        return {"times": times, "errs": errs, "iters": iters, "rep": rep, "x": x,
+                "precond": summary}
 
     def step_plain():
         a, x = step()
         return a, x, a.report
 
-    def step_mixed():
+    def step_mixed(storage=None):
         from mbem_amd.la.mixed import solve_mixed
         a = AssembledDense(system, eps, "direct", jump="calibrated")
-        t0 = time.perf_counter()
-        x, rep = solve_mixed(a, rtol=args.rtol, storage=args.storage,
+        x, rep = solve_mixed(a, rtol=args.rtol, storage=storage or args.storage,
                              inner_rtol=args.inner_rtol, precond_bj_chunk=args.bj_chunk_mixed,
                              precond_grouping=args.grouping_mixed, recycle=args.recycle)
         a.timings.update(mixed_operator_s=rep.operator_build_s,
@@ -682,18 +727,33 @@ def main():
         torch.cuda.synchronize()
     for label, fn in (("plain", step_plain), ("mixed", step_mixed)):
         if args.method in (label, "both"):
-            results[label] = timed(fn, label)
+            results[label] = timed(fn, label, args.steps)
+    other = "bf16" if args.storage != "bf16" else "int8"
+    if args.method in ("mixed", "both") and args.other_storage_steps > 0:
+        results["mixed_" + other] = timed(lambda: step_mixed(other), f"mixed {other}",
+                                          args.other_storage_steps)
     main_label = "mixed" if "mixed" in results else "plain"
-    times, errs, iters, asm, rep_main = results[main_label]
+    times, errs, iters = (results[main_label][k] for k in ("times", "errs", "iters"))
     total = float(np.sum(times))
     ms_per_step = 1e3 * total / args.steps
-    plain_s = float(np.mean(results["plain"][0])) if "plain" in results else None
-    mixed_s = float(np.mean(results["mixed"][0])) if "mixed" in results else None
-    pre = results["plain"][4].precond_summary if "plain" in results else {}
+    plain_s = float(np.mean(results["plain"]["times"])) if "plain" in results else None
+    mixed_s = float(np.mean(results["mixed"]["times"])) if "mixed" in results else None
+    other_s = (float(np.mean(results["mixed_" + other]["times"]))
+               if "mixed_" + other in results else None)
+    pre = (results["plain"]["precond"] or {}) if "plain" in results else {}
+    # the error of each method's last solution against the FP64 reference,
+    # per quantity: displacement (u-slots), traction (t-slots), and the
+    # subtracted stress on the 50 x 50 surface grid, on the top face (z = 0,
+    # half the interior limit there) and 4 eps below it
+    accuracy = {}
+    for label, res in results.items():
+        accuracy[label] = quantity_errors(model, system, eps, res["x"], x_ref)
+        res["x"] = None
+        print(f"[bench] {label} accuracy vs the FP64 reference: {accuracy[label]}", flush=True)
     learned = learned_ab(system, eps, args) if "plain" in results else None
     cpu = None
     if not args.no_cpu_baseline:
-        it_plain = results["plain"][2] if "plain" in results else [60]
+        it_plain = results["plain"]["iters"] if "plain" in results else [60]
         cpu = cpu_baseline(model, eps, system, int(np.median(it_plain)))
         print(f"[bench] cpu baseline {cpu['value']:.1f} s on {cpu['cores']} cores", flush=True)
     def roofline(kernel, ms, nbytes, what, ref=None):
@@ -727,7 +787,7 @@ def main():
         "higher_is_better": False,
         "scaling": "strong",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": (f"f64+{args.storage}" if main_label == "mixed" else "f64"),
         "data": "synthetic box-with-dipping-fault mesh generated in bench.py",
         "config": {
             "workload": f"box 40x40x20 km, N={info['box_triangles']} P0 triangles "
@@ -741,7 +801,11 @@ def main():
             "target_rel_err": args.target_err, "iterations": iters,
             "plain_fp64_solve_s": plain_s, "mixed_solve_s": mixed_s,
             "speedup_vs_plain_fp64": (plain_s / mixed_s if plain_s and mixed_s else None),
-            "plain_rel_err": (max(results["plain"][1]) if "plain" in results else None),
+            "plain_rel_err": (max(results["plain"]["errs"]) if "plain" in results else None),
+            f"mixed_{other}_solve_s": other_s,
+            f"mixed_{other}_rel_err": (max(results["mixed_" + other]["errs"])
+                                       if "mixed_" + other in results else None),
+            "accuracy_vs_fp64_reference": accuracy,
             "precond_build_s": pre.get("build_s"), "precond_chunks": pre.get("chunks"),
             "plain_precond": f"block-Jacobi FP64, {args.grouping} grouping, chunk {args.bj_chunk}",
             "mixed_precond": (f"inner FGMRES to {args.inner_rtol:g} on the mixed operator, "

@@ -137,6 +137,19 @@ def warn_collocation_near_fault(system: BlockSystem, eps):
                       f"row is not corrected")
 
 
+def warn_condition(cond: float) -> bool:
+    """Warn when the dense LU's 1-norm condition estimate is above
+    ``defaults.COND_WARN_THRESHOLD`` (dense.py:555-562): at that size the
+    solve can miss ``defaults.SOLUTION_RTOL``.  Returns whether it warned."""
+    if not cond > defaults.COND_WARN_THRESHOLD:
+        return False
+    warnings.warn(f"dense solve: condition estimate {cond:.2e} is above "
+                  f"{defaults.COND_WARN_THRESHOLD:.0e}, so the solution may miss the "
+                  f"{defaults.SOLUTION_RTOL:.0e} accuracy target (thin panels, near-fluid "
+                  f"material, an un-anchored calibrated jump or a discretization resonance)")
+    return True
+
+
 def translation_basis(layout) -> np.ndarray:
     """Orthonormal rigid-translation basis (n, 3) on the u-slots (dense.py:190-208)."""
     if not layout.has_kind("u"):
@@ -361,7 +374,10 @@ class AssembledDense:
         self.timings["geometry_s"] = time.perf_counter() - t0
         self.timings["assembly_s"] = None
         self._build()
-        self.timings._resolve = lambda: self.assembly.seconds
+        # the resolver holds the report, not ``self``: no reference cycle keeps
+        # a dropped assembly (and its n x n matrix) alive until a full GC
+        rep = self.assembly
+        self.timings._resolve = lambda: rep.seconds
 
     # -- geometry on the device ------------------------------------------
     def _prepare_geometry(self):
@@ -704,6 +720,7 @@ class AssembledDense:
             fs = time.perf_counter() - t0
             self._lu = (fac, fs, self._cond1(fac, anorm), {"how": how, "factor_s": fs,
                                                            "M": LU})
+            warn_condition(self._lu[2])
         from ..la.solver import gemv
         fac, fs, cond, lu_rep = self._lu
         m = fac.m

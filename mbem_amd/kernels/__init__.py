@@ -8,9 +8,11 @@ KERNELS = (KERNEL_T, KERNEL_U, KERNEL_HT, KERNEL_GT)
 DISPLACEMENT_KERNELS = (KERNEL_T, KERNEL_U)
 
 from .basis import (  # noqa: E402
-    TBasis, UBasis, as_eps_array, assemble_t_basis, assemble_t_matrix, assemble_u_basis,
-    assemble_u_matrix, element_sizes, resolve_eps, resolve_patch_eps, t_coeffs, u_coeffs,
+    TBasis, UBasis, as_eps_array, assemble_t_basis, assemble_t_matrix, assemble_traction_matrix,
+    assemble_u_basis, assemble_u_matrix, condense_columns, element_sizes, identity_ids,
+    resolve_eps, resolve_patch_eps, t_coeffs, u_coeffs,
 )
+from .tri_nodal import n_nodes  # noqa: E402
 
 
 def kernel_n_basis(kernel: str) -> int:
@@ -38,4 +40,5 @@ def kernel_coeffs(kernel: str, material):
 __all__ = ["KERNEL_T", "KERNEL_U", "KERNEL_HT", "KERNEL_GT", "KERNELS", "DISPLACEMENT_KERNELS",
            "kernel_n_basis", "kernel_coeffs", "UBasis", "TBasis", "assemble_u_basis",
            "assemble_t_basis", "assemble_u_matrix", "assemble_t_matrix", "u_coeffs", "t_coeffs",
-           "as_eps_array", "element_sizes", "resolve_eps", "resolve_patch_eps"]
+           "as_eps_array", "element_sizes", "resolve_eps", "resolve_patch_eps",
+           "assemble_traction_matrix", "condense_columns", "identity_ids", "n_nodes"]

@@ -124,9 +124,13 @@ class MixedOperator:
             self.S = torch.empty((n, max(self.nseg, 1)), dtype=torch.float32, device=dev)
             hold = (self.nbr, ptr(self.rowptr), ptr(self.col), self.nnzb, ptr(self.vals))
             call("mbem_near_hold", ptr(A), A.stride(0), *hold, 0, stream())
-            call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf, ptr(self.S),
-                 self.nseg, stream())
-            call("mbem_near_hold", ptr(A), A.stride(0), *hold, 1, stream())
+            try:
+                call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
+                     ptr(self.S), self.nseg, stream())
+            finally:
+                # the near blocks go back into the caller's A whatever the
+                # cast returned: A is never left with them zeroed
+                call("mbem_near_hold", ptr(A), A.stride(0), *hold, 1, stream())
             self.far_bytes = n * n + n * self.nseg * 4
         else:
             dt = torch.int16 if self.bf16 else torch.float32

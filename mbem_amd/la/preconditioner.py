@@ -24,6 +24,7 @@ outer iteration absorbs (the outer residual is the true one).
 from __future__ import annotations
 
 import time
+import weakref
 
 import numpy as np
 import torch
@@ -162,7 +163,11 @@ class BlockGaussSeidel:
         self.bj_chunk = int(bj_chunk)
         self.above_dense = above_dense
         self.grouping = grouping
-        self.asm = assembled
+        # the assembly is held weakly: it owns this preconditioner
+        # (``_precond``), so a strong reference would make a cycle that keeps
+        # its n x n matrix alive after the caller drops it
+        self._asm = weakref.ref(assembled)
+        self.n_unknowns = assembled.layout.n_unknowns
         A = assembled.A_dev
         self.A = A
         n = A.shape[0]
@@ -352,7 +357,7 @@ class BlockGaussSeidel:
     def inverse_diagonal(self) -> torch.Tensor:
         """diag(M) as an FP64 device vector over the unknowns: the diagonal of
         each chunk inverse scattered to its DOFs."""
-        n = self.asm.layout.n_unknowns
+        n = self.n_unknowns
         d = torch.zeros(n, dtype=F64, device=self.inv.device)
         cptr = self.chunk_ptr.cpu().numpy()
         iptr = self.inv_ptr.cpu().numpy()
@@ -362,6 +367,12 @@ class BlockGaussSeidel:
             M = self.inv[int(iptr[c]):int(iptr[c]) + m * ld].view(m, ld)[:, :m]
             d[self.idx[int(cptr[c]):int(cptr[c + 1])]] = M.diagonal().to(F64)
         return d
+
+    @property
+    def asm(self):
+        """The assembly this preconditioner was built from (None once it is
+        gone)."""
+        return self._asm()
 
     def summary(self) -> dict:
         """preconditioner.py:502-519."""

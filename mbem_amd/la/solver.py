@@ -109,19 +109,38 @@ def _solve_upper(R, g):
     return y
 
 
-def _as_op(A):
+def _as_op(A, host: bool = False):
     """Operator ``op(v, out)`` writing A v into ``out`` (solver.py:513-520):
-    a device matrix is applied with the library's GEMV, an object with
-    ``apply_into(v, out)`` writes in place, and ``.matvec`` objects or plain
-    callables return a new vector that is copied in."""
+    a device matrix (or a host ndarray, uploaded once) is applied with the
+    library's GEMV, an object with ``apply_into(v, out)`` writes in place,
+    and ``.matvec`` objects, plain callables and objects with ``@`` return a
+    new vector that is copied in.  With ``host`` (the caller handed a numpy
+    right-hand side: the reference's own callers) those callables see numpy
+    vectors and may return numpy or torch."""
     if A is None:
         return None
+    if isinstance(A, np.ndarray):
+        A = torch.as_tensor(np.ascontiguousarray(A, dtype=np.float64), device=_lib.device())
     if isinstance(A, torch.Tensor):
+        A = A.to(device=_lib.device(), dtype=F64).contiguous()
         return lambda v, out: gemv(A, v, out)
     if hasattr(A, "apply_into"):
         return A.apply_into
-    f = A.matvec if hasattr(A, "matvec") else A
-    return lambda v, out: out.copy_(f(v))
+    if callable(A) and not hasattr(A, "matvec"):
+        f = A
+    elif hasattr(A, "matvec"):
+        f = A.matvec
+    else:
+        f = lambda v: A @ v  # noqa: E731
+    if not host:
+        return lambda v, out: out.copy_(f(v))
+
+    def op(v, out):
+        y = f(v.cpu().numpy())
+        out.copy_(torch.as_tensor(np.asarray(y, dtype=np.float64)) if not
+                  isinstance(y, torch.Tensor) else y)
+        return out
+    return op
 
 
 def gemv(A: torch.Tensor, x: torch.Tensor, out=None, alpha=1.0, beta=0.0):
@@ -190,14 +209,37 @@ def fgmres(A, b, M=None, x0=None,
            stagnation_window: int = defaults.GMRES_STAGNATION_WINDOW,
            stagnation_factor: float = defaults.GMRES_STAGNATION_FACTOR,
            callback=None, recycle=None):
-    """Solve A x = b on the device; returns (x device tensor, SolveReport).
+    """Solve A x = b on the device; returns (x, SolveReport).
 
-    ``A`` and ``M`` are device matrices or callables ``f(v) -> A v`` on
-    device vectors; ``M`` approximates A^-1 (right preconditioner)."""
+    ``A`` and ``M`` are matrices (device tensors or host ndarrays), objects
+    with ``apply_into`` / ``matvec`` / ``@``, or callables ``f(v) -> A v``;
+    ``M`` approximates A^-1 (right preconditioner).  A torch ``b`` gives a
+    device ``x`` and callables see device vectors; a numpy ``b`` (the
+    reference's callers) gives a numpy ``x`` and callables see numpy
+    vectors.  A complex ``b`` raises TypeError, as in the reference."""
     _lib.require_gpu()
-    A_mv = _as_op(A)
-    M_mv = _as_op(M) if M is not None else None
-    b = b if isinstance(b, torch.Tensor) else torch.as_tensor(np.asarray(b, float))
+    host = not isinstance(b, torch.Tensor)
+    if host:
+        b = np.asarray(b)
+        if np.iscomplexobj(b):
+            raise TypeError("fgmres is real-only")
+        b = torch.as_tensor(np.asarray(b, dtype=np.float64))
+    elif b.is_complex():
+        raise TypeError("fgmres is real-only")
+    if host:
+        x, rep = _fgmres_device(_as_op(A, True), _as_op(M, True) if M is not None else None, b,
+                                x0, rtol, restart, maxiter, stagnation_window, stagnation_factor,
+                                callback, recycle)
+        return x.cpu().numpy(), rep
+    return _fgmres_device(_as_op(A), _as_op(M) if M is not None else None, b, x0, rtol,
+                          restart, maxiter, stagnation_window, stagnation_factor, callback,
+                          recycle)
+
+
+def _fgmres_device(A_mv, M_mv, b, x0, rtol, restart, maxiter, stagnation_window,
+                   stagnation_factor, callback, recycle):
+    """The iteration of ``fgmres`` on device vectors, with ``A_mv`` / ``M_mv``
+    already in the ``op(v, out)`` form."""
     b = b.to(device=_lib.device(), dtype=F64).contiguous()
     n = b.shape[0]
     dev = b.device
@@ -205,8 +247,10 @@ def fgmres(A, b, M=None, x0=None,
     b_norm = ws.norm(b)
     if b_norm == 0.0:
         return torch.zeros(n, dtype=F64, device=dev), SolveReport(True, 0, 0.0, 0.0)
+    if x0 is not None and not isinstance(x0, torch.Tensor):
+        x0 = torch.as_tensor(np.asarray(x0, dtype=np.float64))
     x = torch.zeros(n, dtype=F64, device=dev) if x0 is None else \
-        torch.as_tensor(x0, dtype=F64, device=dev).clone().contiguous()
+        x0.to(device=dev, dtype=F64).clone().contiguous()
     if recycle is not None:
         # GCRO-DR with the subspace carried in ``recycle`` (solver.py:248-395)
         from .gcrodr import GcroDR
