@@ -69,13 +69,17 @@ def i8_bytes(n: int) -> int:
     return n * n + 4 * n * (-(-n // 64)) + 4 * n + 8 * n
 
 
-def _release():
-    """Free what the last step left: collect garbage first (a dropped
-    object still in a reference cycle keeps its device memory until then),
-    then return the cached blocks to the device."""
+def _release(full: bool = False):
+    """Free what the last step left: collect garbage (a dropped object still
+    in a reference cycle keeps its device memory until then); with ``full``
+    also return the allocator's cached blocks to the device.  Between steps
+    of one method the blocks stay cached, so each step reuses the last one's
+    (as any caller solving repeatedly does) instead of mapping 28.8 GB
+    afresh; between methods and phases the cache is emptied."""
     import torch
     gc.collect()
-    torch.cuda.empty_cache()
+    if full:
+        torch.cuda.empty_cache()
 
 
 def quantity_errors(model, system, eps, x, x_ref):
@@ -324,7 +328,7 @@ def bench_laplace(args):
             ref = xs[r].cpu().numpy()
             errs.append(float(np.linalg.norm(x[i, r] - ref) / np.linalg.norm(ref)))
         del asm, M
-        _release()
+        _release(full=True)
         print(f"[laplace] sequential state {st.label}: {seq[-1]:.3f} s, "
               f"max rel diff vs batched {max(errs[-len(slips):]):.2e}", flush=True)
     seq_est = float(np.mean(seq)) * len(states)
@@ -484,7 +488,7 @@ def learned_ab(system, eps, args):
     print(f"[bench] learned A/B: {out}", flush=True)
     asm.release()
     asm.A_dev = None
-    _release()
+    _release(full=True)
     return out
 
 
@@ -681,7 +685,7 @@ def main():
               f"{ref_bytes / ref_ms / 1e6:.0f} GB/s", flush=True)
         del op
     asm = None
-    _release()
+    _release(full=True)
     print(f"[bench] gemv {gemv_ms:.3f} ms = {gemv_bytes / gemv_ms / 1e6:.0f} GB/s setup "
           f"{time.perf_counter() - t_setup:.1f} s", flush=True)
 
@@ -691,7 +695,7 @@ def main():
         torch.cuda.synchronize()
         for k in range(steps):
             asm_k = x = None
-            _release()
+            _release(full=k == 0)
             ts = time.perf_counter()
             asm_k, x, rep = step_fn()
             torch.cuda.synchronize()

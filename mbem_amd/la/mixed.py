@@ -361,6 +361,20 @@ class MixedReport:
                 f"inner iterations {self.inner_iterations}")
 
 
+_SIDE = {}
+
+
+def _side_stream():
+    """The side stream of the operator build, one per device for the life of
+    the process (default priority).  A new stream per solve would leave the
+    caching allocator's blocks of the last build (held per stream) unusable
+    by the next, so every solve would map its far-field copy afresh."""
+    dev = torch.cuda.current_device()
+    if dev not in _SIDE:
+        _SIDE[dev] = torch.cuda.Stream(device=dev)
+    return _SIDE[dev]
+
+
 def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
                 inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
                 outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=1500,
@@ -379,7 +393,7 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     # inversions, which work on their own copies of the chunks.
     t0 = time.perf_counter()
     main = torch.cuda.current_stream()
-    side = torch.cuda.Stream()
+    side = _side_stream()
     near = None
     if pattern is None:
         with torch.cuda.stream(side):

@@ -4,7 +4,8 @@ path: device assembly, FGMRES, and the device contractions.
 
 * Somigliana's identity with exact densities: the same bars as the oracle
   (interior stress 1e-3, displacement 5e-4; half on the face; zero outside)
-  and the oracle's readout of the same densities to 1e-10;
+  and the oracle's readout of the same densities to 1e-10 of the field's
+  size;
 * B1 rigid translation, B8 uniform strain with a trace and B7 half values
   on a face, each solved on the device: the reference's bars
   (verify_solved_bvp.py:907-935, 1188-1237, 1266-1290), and the readout
@@ -41,8 +42,11 @@ def test_somigliana_identity_gpu():
         uo = oe.evaluate_displacement(model, region, sol, q, eps)
         ue = B.RIGID_C + q @ B.STRAIN.T
         scale = np.linalg.norm(ue, axis=1).max()
-        print(f"{name}: vs oracle {_rel(s, so):.1e} / {_rel(u, uo):.1e}")
-        assert _rel(s, so) < 1e-10 and _rel(u, uo) < 1e-10
+        # against the oracle relative to the field's own size (outside the
+        # body both readouts are cancellations near zero)
+        ds, du = np.abs(s - so).max() / sx, np.abs(u - uo).max() / scale
+        print(f"{name}: vs oracle {ds:.1e} / {du:.1e}")
+        assert ds < 1e-10 and du < 1e-10
         if name == "in":
             assert B.tensor_error(s, mat) < 1e-3
             assert np.abs(u - ue).max() / scale < 5e-4
