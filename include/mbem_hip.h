@@ -35,6 +35,9 @@ extern "C" {
 #define MBEM_OK 0
 #define MBEM_EINVAL 1
 #define MBEM_EHIP 2
+/* the call needs more device scratch than the library holds
+ * (mbem_multidot_noalloc) */
+#define MBEM_ESCRATCH 3
 
 /* rows of a block-Jacobi chunk one wavefront applies (mbem_bj_apply tables) */
 #define MBEM_BJ_ROWS_PER_WAVE 4
@@ -136,6 +139,18 @@ int mbem_multidot(const double *V, int64_t ldv, int64_t k, int64_t n, const doub
                   double *work, hipStream_t stream);
 int mbem_multiaxpy(const double *V, int64_t ldv, int64_t k, int64_t n, const double *h, double *w,
                    double alpha, hipStream_t stream);
+
+/* h = V w as mbem_multidot, allocating nothing: the partial sums go to a
+ * device array the library holds, MBEM_NOALLOC_SCRATCH_DOUBLES doubles
+ * (8 MiB) on each GPU it runs on, until the process ends.  No counterpart in
+ * the reference: it is for callers that cannot allocate (code under HIP
+ * graph capture, tight serving loops).  Limit: ceil(n / 512) * k <=
+ * MBEM_NOALLOC_SCRATCH_DOUBLES, else MBEM_ESCRATCH; the first call on a
+ * device looks up that device's copy.  Calls that overlap on one GPU race on
+ * the array; results equal mbem_multidot's bit for bit. */
+#define MBEM_NOALLOC_SCRATCH_DOUBLES (1 << 20)
+int mbem_multidot_noalloc(const double *V, int64_t ldv, int64_t k, int64_t n, const double *w,
+                          double *h, hipStream_t stream);
 
 /* y = a x + b y */
 int mbem_axpby(int64_t n, double a, const double *x, double b, double *y, hipStream_t stream);

@@ -17,7 +17,8 @@ import torch  # noqa: F401  (must precede the library load, see docstring)
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmbem_hip.so")
 
-MBEM_OK, MBEM_EINVAL, MBEM_EHIP = 0, 1, 2
+MBEM_OK, MBEM_EINVAL, MBEM_EHIP, MBEM_ESCRATCH = 0, 1, 2, 3
+NOALLOC_SCRATCH_DOUBLES = 1 << 20      # MBEM_NOALLOC_SCRATCH_DOUBLES
 BJ_ROWS_PER_WAVE = 4
 BJ32_ROWS_PER_WAVE = 8
 
@@ -44,6 +45,7 @@ _SIGS = {
     "mbem_gemv": [_p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
     "mbem_multidot": [_p, _i64, _i64, _i64, _p, _p, _p, _p],
     "mbem_multiaxpy": [_p, _i64, _i64, _i64, _p, _p, _d, _p],
+    "mbem_multidot_noalloc": [_p, _i64, _i64, _i64, _p, _p, _p],
     "mbem_axpby": [_i64, _d, _p, _d, _p, _p],
     "mbem_normalize": [_i64, _p, _p, _p],
     "mbem_gather_block": [_p, _i64, _p, _i64, _p, _p],
@@ -197,7 +199,9 @@ def ptr(t) -> ctypes.c_void_p:
 def check(status: int, what: str) -> None:
     if status == MBEM_OK:
         return
-    kind = {MBEM_EINVAL: "invalid argument", MBEM_EHIP: "HIP error"}.get(status, f"status {status}")
+    kind = {MBEM_EINVAL: "invalid argument", MBEM_EHIP: "HIP error",
+            MBEM_ESCRATCH: "needs more than the library's fixed device scratch"}.get(
+        status, f"status {status}")
     raise MbemError(f"{what}: {kind}")
 
 

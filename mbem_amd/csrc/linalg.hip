@@ -12,6 +12,8 @@
 #include <rocsolver/rocsolver.h>
 #include <stdint.h>
 
+#include <mutex>
+
 #include "../../include/mbem_hip.h"
 
 namespace {
@@ -530,6 +532,52 @@ int mbem_multidot(const double* V, int64_t ldv, int64_t k, int64_t n, const doub
     int64_t parts = (n + kDotChunk - 1) / kDotChunk;
     if (parts == 0) parts = 1;
     if (parts > 65535) return MBEM_EINVAL;      // grid.y
+    const int vec = (ldv % 2 == 0) && (reinterpret_cast<uintptr_t>(V) % 16 == 0) &&
+                    (reinterpret_cast<uintptr_t>(w) % 16 == 0);
+    dim3 grid((unsigned)((k + 3) / 4), (unsigned)parts), block(256);
+    hipLaunchKernelGGL(k_multidot_part, grid, block, 0, stream, V, ldv, k, n, w, work, vec);
+    hipLaunchKernelGGL(k_reduce_parts, dim3((unsigned)((k + 3) / 4)), dim3(256), 0, stream, work, k, parts, h);
+    return launch_status();
+}
+
+// Scratch of mbem_multidot_noalloc: one call's partial sums, parts x k
+// doubles (parts = ceil(n / kDotChunk)), in a fixed device array so that the
+// call allocates nothing.  Each GPU holds its own copy until the process
+// ends; its address on a device is read once per device (hipGetSymbolAddress
+// with that device current) and kept: the array's name in host code is not a
+// device address.  Every entry a call reads is written by the same call
+// before it is read, so nothing needs clearing between calls; calls that
+// overlap on one GPU (two streams, or two host threads) race on it.
+static __device__ __align__(256) double scratch[MBEM_NOALLOC_SCRATCH_DOUBLES];
+
+constexpr int kMaxDevices = 64;
+
+static double* scratch_on_current_device()
+{
+    static std::mutex mu;
+    static double* addr[kMaxDevices] = {};
+    int dev = -1;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[dev]) {
+        void* p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return nullptr;
+        addr[dev] = static_cast<double*>(p);
+    }
+    return addr[dev];
+}
+
+int mbem_multidot_noalloc(const double* V, int64_t ldv, int64_t k, int64_t n, const double* w,
+                          double* h, hipStream_t stream)
+{
+    if (k < 0 || n < 0 || (k > 0 && (!V || !h || ldv < n)) || (n > 0 && !w)) return MBEM_EINVAL;
+    if (k == 0) return MBEM_OK;
+    int64_t parts = (n + kDotChunk - 1) / kDotChunk;
+    if (parts == 0) parts = 1;
+    if (parts > 65535) return MBEM_EINVAL;      // grid.y
+    if (parts > MBEM_NOALLOC_SCRATCH_DOUBLES / k) return MBEM_ESCRATCH;
+    double* work = scratch_on_current_device();
+    if (!work) return MBEM_EHIP;
     const int vec = (ldv % 2 == 0) && (reinterpret_cast<uintptr_t>(V) % 16 == 0) &&
                     (reinterpret_cast<uintptr_t>(w) % 16 == 0);
     dim3 grid((unsigned)((k + 3) / 4), (unsigned)parts), block(256);
