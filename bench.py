@@ -582,6 +582,8 @@ def main():
     ap.add_argument("--inner-rtol", type=float, default=5e-2)
     ap.add_argument("--bj-chunk-mixed", type=int, default=1500)
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
+    ap.add_argument("--coarse", choices=("none", "const", "rigid", "linear"), default="rigid",
+                    help="coarse space of the mixed method's two-level block-Jacobi")
     ap.add_argument("--other-storage-steps", type=int, default=3,
                     help="steps of the mixed method with the other far-field storage (BF16 "
                          "beside int8), reported beside the headline")
@@ -720,7 +722,8 @@ def main():
         a = AssembledDense(system, eps, "direct", jump="calibrated")
         x, rep = solve_mixed(a, rtol=args.rtol, storage=storage or args.storage,
                              inner_rtol=args.inner_rtol, precond_bj_chunk=args.bj_chunk_mixed,
-                             precond_grouping=args.grouping_mixed, recycle=args.recycle)
+                             precond_grouping=args.grouping_mixed, recycle=args.recycle,
+                             coarse=(None if args.coarse == "none" else args.coarse))
         a.timings.update(mixed_operator_s=rep.operator_build_s,
                          precond_s=rep.precond_build_s, fgmres_s=rep.solve_s)
         return a, x, rep
@@ -814,8 +817,9 @@ def main():
             "plain_precond": f"block-Jacobi FP64, {args.grouping} grouping, chunk {args.bj_chunk}",
             "mixed_precond": (f"inner FGMRES to {args.inner_rtol:g} on the mixed operator, "
                               f"block-Jacobi FP32, {args.grouping_mixed} grouping, chunk "
-                              f"{args.bj_chunk_mixed}, {args.recycle} recycled vectors across "
-                              f"inner solves"),
+                              f"{args.bj_chunk_mixed}, coarse space {args.coarse}, "
+                              f"{args.recycle} recycled vectors across inner solves"),
+            "mixed_coarse": (results["mixed"]["rep"].coarse if "mixed" in results else None),
             "learned_preconditioner": learned,
         },
         "roofline": roof_main,

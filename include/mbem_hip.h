@@ -355,6 +355,11 @@ int mbem_gemm_g(const double *A0, const double *A1, int64_t lda, int64_t m, int6
 int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
                         const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
                         hipStream_t stream);
+/* C (m x n) = alpha A B^T + beta C, A (m x k) and B (n x k) row-major (the
+ * coarse space's Gram matrices). */
+int mbem_dgemm_abt(int64_t m, int64_t n, int64_t k, double alpha, const double *A, int64_t lda,
+                   const double *B, int64_t ldb, double beta, double *C, int64_t ldc,
+                   hipStream_t stream);
 
 /* Grouped GEMM over a device work table of nwork items (mbem_work_item_bytes
  * each: chunk inverse offset and leading dimension, chunk size, block offset,
@@ -482,6 +487,45 @@ int mbem_lr_outer(const float *U, int64_t ldu, const double *C, int64_t B, int64
                   int64_t off, float *dY, hipStream_t stream);
 int mbem_lr_loss(const float *pred, const float *tgt, int64_t n, int64_t B, float *dpred,
                  float *loss_b, hipStream_t stream);
+
+/* Coarse space of the two-level block-Jacobi preconditioner (an addition to
+ * the reference's ladder, la/preconditioner.py:421-470, used by the
+ * mixed-precision solve): W (n x G q) has q shape functions (q = 3, 6 or 12)
+ * on each group g of DOFs, the groups' DOFs listed in perm (group g at
+ * perm[gptr[g] .. gptr[g+1])) with values Wp[t q + j].
+ * mbem_coarse_image: Ct = (A W)^T, (G q) x m row-major with leading
+ * dimension ldc >= m, one pass over the m rows of A (row-major, lda).
+ * mbem_coarse_expand: x[perm[t]] += alpha sum_j Wp[t q + j] u[grp[t] q + j]
+ * for the nt listed DOFs. */
+int mbem_coarse_image(const double *A, int64_t lda, int64_t m, const int64_t *perm,
+                      const double *Wp, const int64_t *gptr, int64_t G, int q, double *Ct,
+                      int64_t ldc, hipStream_t stream);
+/* The coarse image of the mixed operator F + N (la/mixed.py): Ct = (F W)^T
+ * from the int8 far field (codes, FP32 scales per row and 64 columns, near
+ * entries zero; overwrites Ct), then Ct += (N W)^T from the near field's
+ * FP64 3 x 3 block-CSR; inv[d] is the listed position of DOF d (the inverse
+ * of perm), grp[t] the group of listed position t. */
+int mbem_coarse_image_i8(const uint8_t *F, int64_t ldf, const float *scale, int64_t nseg, int64_t m,
+                         const int64_t *perm, const double *Wp, const int64_t *gptr, int64_t G, int q,
+                         double *Ct, int64_t ldc, hipStream_t stream);
+int mbem_coarse_image_bsr(int64_t nbr, const int64_t *rowptr, const int64_t *col, const double *vals,
+                          const int64_t *inv, const int64_t *grp, const double *Wp, int q, double *Ct,
+                          int64_t ldc, hipStream_t stream);
+/* The coarse space's factors from its Gram matrix G (K x K, overwritten),
+ * on the device with no host wait: D = diag(sqrt(G_ii)) (d, K doubles),
+ * Cholesky L L^T = D^-1 G D^-1 + delta I (rocSOLVER potrf), L^-1 (trtri),
+ * then Tt = L^-1 D^-1 and T = Tt^T (both K x K row-major), so that Q = C T
+ * has orthonormal columns for C^T C = G.  info: 2 ints (potrf, trtri). */
+int mbem_coarse_factor(double *G, int64_t K, double delta, double *d, int32_t *info, double *Tt,
+                       double *T, hipStream_t stream);
+/* G = X X^T (k x k) for X (k x n) row-major with leading dimension ldx,
+ * split-K on matrix cores (strided-batched DGEMM over n-slices, then a sum):
+ * the coarse space's Gram matrix.  work: mbem_gram_work(k, n) doubles. */
+int64_t mbem_gram_work(int64_t k, int64_t n);
+int mbem_gram(const double *X, int64_t ldx, int64_t k, int64_t n, double *G, double *work,
+              hipStream_t stream);
+int mbem_coarse_expand(const double *u, const int64_t *perm, const double *Wp, const int64_t *grp,
+                       int64_t nt, int q, double alpha, double *x, hipStream_t stream);
 
 #ifdef __cplusplus
 }

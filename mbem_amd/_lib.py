@@ -89,6 +89,7 @@ _SIGS = {
     "mbem_gemm_g": [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _int, _p],
     "mbem_work_item_bytes": [],
     "mbem_dgemm_rowmajor": [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p],
+    "mbem_dgemm_abt": [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p],
     "mbem_gemm_grouped": [_p, _p, _i64, _p, _i64, _p, _i64, _int, _p],
     "mbem_bdot": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _p],
     "mbem_breduce": [_p, _i64, _i64, _i64, _p, _i64, _i64, _int, _p, _p],
@@ -107,6 +108,14 @@ _SIGS = {
     "mbem_graph_launch": [_p, _p],
     "mbem_graph_destroy": [_p],
     "mbem_memcpy_async": [_p, _p, _i64, _p],
+    # coarse space of the two-level block-Jacobi
+    "mbem_coarse_image": [_p, _i64, _i64, _p, _p, _p, _i64, _int, _p, _i64, _p],
+    "mbem_coarse_expand": [_p, _p, _p, _p, _i64, _int, _d, _p, _p],
+    "mbem_gram_work": [_i64, _i64],
+    "mbem_coarse_image_i8": [_p, _i64, _p, _i64, _i64, _p, _p, _p, _i64, _int, _p, _i64, _p],
+    "mbem_coarse_image_bsr": [_i64, _p, _p, _p, _p, _p, _p, _int, _p, _i64, _p],
+    "mbem_gram": [_p, _i64, _i64, _i64, _p, _p, _p],
+    "mbem_coarse_factor": [_p, _i64, _d, _p, _p, _p, _p, _p],
     # learned preconditioner
     "mbem_knn": [_p, _i64, _int, _p, _p],
     "mbem_sgemm": [_p, _i64, _i64, _p, _i64, _i64, _p, _i64, _i64, _i64, _i64, _f, _f, _p, _int,
@@ -126,7 +135,7 @@ _SIGS = {
     "mbem_lr_outer": [_p, _i64, _p, _i64, _i64, _i64, _i64, _p, _p],
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }
-_RESTYPE = {"mbem_multidot_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
+_RESTYPE = {"mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
 
 _lib = None
 

@@ -589,7 +589,8 @@ class AssembledDense:
               x0=None, precond_max_dense=None,
               precond_above_dense: str = defaults.PRECOND_RUNG_ABOVE_DENSE,
               precond_hodlr_max=None, precond_bj_chunk=None, recycle=None,
-              return_device: bool = False, precond_grouping: str = defaults.PRECOND_GROUPING):
+              return_device: bool = False, precond_grouping: str = defaults.PRECOND_GROUPING,
+              precond_coarse: str = None):
         """{slot name: (n_dofs, 3)}; leaves ``self.report`` (SolveReport for
         FGMRES, DenseSolveReport for LU).  ``precond_grouping`` ("patch": a
         Gauss-Seidel sweep over patches, the reference's default; "region":
@@ -603,7 +604,11 @@ class AssembledDense:
         ``defaults.GCRO_RECYCLE_DEFAULT``) carries a GCRO-DR subspace through
         it, dropped when the materials have moved more than
         ``defaults.PRECOND_REUSE_MAX_STEP`` since it was harvested.  A
-        ``RecycleSpace`` may also be passed to carry one explicitly."""
+        ``RecycleSpace`` may also be passed to carry one explicitly.
+
+        ``precond_coarse`` ("const", "rigid" or "linear"; an addition to
+        the reference's keywords) puts the coarse correction of
+        la/coarse.py over the block preconditioner."""
         lineage = self._lineage
         if isinstance(x0, str):
             if x0 != "previous":
@@ -630,7 +635,7 @@ class AssembledDense:
         elif method == "fgmres":
             x = self._solve_fgmres(rtol, restart, maxiter, x0, precond_max_dense,
                                    precond_above_dense, precond_bj_chunk, recycle,
-                                   precond_grouping)
+                                   precond_grouping, precond_coarse)
         else:
             raise ValueError(f"solve method {method!r} (expected 'fgmres' or 'lu')")
         self.x_dev = x
@@ -642,22 +647,26 @@ class AssembledDense:
 
     def preconditioner(self, max_dense=None, bj_chunk=None,
                        above_dense=defaults.PRECOND_RUNG_ABOVE_DENSE,
-                       grouping=defaults.PRECOND_GROUPING):
+                       grouping=defaults.PRECOND_GROUPING, coarse=None):
         from ..la.preconditioner import BlockGaussSeidel
-        caps = (max_dense, above_dense, bj_chunk, grouping)
+        caps = (max_dense, above_dense, bj_chunk, grouping, coarse)
         if self._precond is not None and self._precond_caps != caps:
             self._precond = None
         if self._precond is None:
-            self._precond = BlockGaussSeidel(self, max_dense=max_dense, bj_chunk=bj_chunk,
-                                             above_dense=above_dense, grouping=grouping)
+            M = BlockGaussSeidel(self, max_dense=max_dense, bj_chunk=bj_chunk,
+                                 above_dense=above_dense, grouping=grouping)
+            if coarse:
+                from ..la.coarse import CoarseSpace, TwoLevel
+                M = TwoLevel(M, CoarseSpace(self, M.chunk_dofs, basis=coarse))
+            self._precond = M
             self._precond_caps = caps
         return self._precond
 
     def _solve_fgmres(self, rtol, restart, maxiter, x0, max_dense, above_dense, bj_chunk,
-                      recycle, grouping=defaults.PRECOND_GROUPING):
+                      recycle, grouping=defaults.PRECOND_GROUPING, coarse=None):
         from ..la.solver import fgmres
         t0 = time.perf_counter()
-        M = self.preconditioner(max_dense, bj_chunk, above_dense, grouping)
+        M = self.preconditioner(max_dense, bj_chunk, above_dense, grouping, coarse)
         self.timings["precond_s"] = time.perf_counter() - t0
         t0 = time.perf_counter()
         if x0 is not None and not isinstance(x0, torch.Tensor):

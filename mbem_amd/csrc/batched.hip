@@ -21,6 +21,7 @@
 // Block vectors are row-major n x K (ld = K): column c is system c.
 #include <hip/hip_runtime.h>
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 #include <stdint.h>
 
 #include "../../include/mbem_hip.h"
@@ -486,6 +487,61 @@ int dispatch(int cfg, dim3 grid, hipStream_t st, const double* A0, const double*
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
 
+// out[i] = sum_s part[s * len + i]: the split-K partial Gram matrices summed
+// in a fixed order (one thread per entry)
+__global__ __launch_bounds__(256) void k_sum_slices(const double* __restrict__ part, long len,
+                                                    long S, double* __restrict__ out)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= len) return;
+    double acc = 0.0;
+    for (long s = 0; s < S; ++s) acc += part[s * len + i];
+    out[i] = acc;
+}
+
+// The coarse space's Gram matrix G (K x K, symmetric) scaled to unit
+// diagonal and regularised: d_i = sqrt(G_ii) (1 for an all-zero column,
+// whose diagonal becomes 1), G_ij <- G_ij / (d_i d_j) + delta [i = j].
+// One thread per entry; d is written by the diagonal's threads first (a
+// separate launch), so every entry reads finished d values.
+__global__ __launch_bounds__(256) void k_gram_diag(const double* __restrict__ G, long K,
+                                                   double* __restrict__ d)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= K) return;
+    const double g = G[i * K + i];
+    d[i] = g > 0.0 ? sqrt(g) : 0.0;
+}
+
+__global__ __launch_bounds__(256) void k_gram_scale(double* __restrict__ G, long K,
+                                                    const double* __restrict__ d, double delta)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K * K) return;
+    const long i = e / K, j = e - i * K;
+    const double di = d[i] > 0.0 ? d[i] : 1.0, dj = d[j] > 0.0 ? d[j] : 1.0;
+    double v = G[e] / (di * dj);
+    if (i == j) v = (d[i] > 0.0 ? v : 1.0) + delta;
+    G[e] = v;
+}
+
+// From M, whose column-major lower triangle holds Linv = L^-1 (the row-major
+// view: M[j * K + i] = Linv(i, j), i >= j), form the row-major
+// Tt = Linv D^-1 and T = Tt^T, D = diag(d) (1 for a zero column).
+__global__ __launch_bounds__(256) void k_coarse_tfactors(const double* __restrict__ M, long K,
+                                                         const double* __restrict__ d,
+                                                         double* __restrict__ Tt,
+                                                         double* __restrict__ T)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= K * K) return;
+    const long i = e / K, j = e - i * K;
+    const double dj = d[j] > 0.0 ? d[j] : 1.0;
+    const double v = (i >= j) ? M[j * K + i] / dj : 0.0;
+    Tt[i * K + j] = v;
+    T[j * K + i] = v;
+}
+
 }  // namespace
 
 extern "C" {
@@ -547,6 +603,83 @@ int mbem_dgemm_rowmajor(int64_t m, int64_t n, int64_t k, double alpha, const dou
     const rocblas_status st = rocblas_dgemm_64(g_blas, rocblas_operation_none, rocblas_operation_none,
                                                n, m, k, &alpha, B, ldb, A, lda, &beta, C, ldc);
     return st == rocblas_status_success ? launch_status() : MBEM_EHIP;
+}
+
+int mbem_dgemm_abt(int64_t m, int64_t n, int64_t k, double alpha, const double* A, int64_t lda,
+                   const double* B, int64_t ldb, double beta, double* C, int64_t ldc,
+                   hipStream_t stream)
+{
+    if (m < 0 || n < 0 || k < 0 || (m > 0 && n > 0 && (!A || !B || !C)) || lda < k || ldb < k || ldc < n)
+        return MBEM_EINVAL;
+    if (m == 0 || n == 0) return MBEM_OK;
+    if (!g_blas && rocblas_create_handle(&g_blas) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_blas, stream) != rocblas_status_success) return MBEM_EHIP;
+    // row-major C (m x n) = A (m x k) B^T with B (n x k) row-major  <=>
+    // column-major C^T = B A^T, B read transposed from its column-major view
+    const rocblas_status st = rocblas_dgemm_64(g_blas, rocblas_operation_transpose, rocblas_operation_none,
+                                               n, m, k, &alpha, B, ldb, A, lda, &beta, C, ldc);
+    return st == rocblas_status_success ? launch_status() : MBEM_EHIP;
+}
+
+int64_t mbem_gram_work(int64_t k, int64_t n)
+{
+    if (k <= 0 || n <= 0) return 1;
+    const int64_t S = n / 1024 > 0 ? (n / 1024 < 128 ? n / 1024 : 128) : 1;
+    return S * k * k;
+}
+
+int mbem_gram(const double* X, int64_t ldx, int64_t k, int64_t n, double* G, double* work,
+              hipStream_t stream)
+{
+    if (k < 0 || n < 0 || ldx < n || (k > 0 && (!X || !G || !work))) return MBEM_EINVAL;
+    if (k == 0) return MBEM_OK;
+    if (!g_blas && rocblas_create_handle(&g_blas) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_blas, stream) != rocblas_status_success) return MBEM_EHIP;
+    const double one = 1.0, zero = 0.0;
+    // X (k x n row-major) is column-major (n x k) with leading dimension ldx:
+    // G = X X^T is that matrix's (k x k) Gram.  Split the n-long inner
+    // dimension into S equal slices, one strided-batched DGEMM over them
+    // (each slice a k x k partial), summed; the remainder columns are added
+    // by one more GEMM with beta = 1.
+    const int64_t S = n / 1024 > 0 ? (n / 1024 < 128 ? n / 1024 : 128) : 1;
+    const int64_t L = n / S;
+    rocblas_status st = rocblas_dgemm_strided_batched_64(
+        g_blas, rocblas_operation_transpose, rocblas_operation_none, k, k, L, &one, X, ldx, L, X, ldx,
+        L, &zero, work, k, k * k, S);
+    if (st != rocblas_status_success) return MBEM_EHIP;
+    const long len = (long)(k * k);
+    hipLaunchKernelGGL(k_sum_slices, dim3((unsigned)((len + 255) / 256)), dim3(256), 0, stream, work,
+                       len, (long)S, G);
+    if (S * L < n) {
+        st = rocblas_dgemm_64(g_blas, rocblas_operation_transpose, rocblas_operation_none, k, k,
+                              n - S * L, &one, X + S * L, ldx, X + S * L, ldx, &one, G, k);
+        if (st != rocblas_status_success) return MBEM_EHIP;
+    }
+    return launch_status();
+}
+
+int mbem_coarse_factor(double* G, int64_t K, double delta, double* d, int32_t* info, double* Tt,
+                       double* T, hipStream_t stream)
+{
+    if (K < 0 || K > INT32_MAX || (K > 0 && (!G || !d || !info || !Tt || !T)) || !(delta > 0.0))
+        return MBEM_EINVAL;
+    if (K == 0) return MBEM_OK;
+    if (!g_blas && rocblas_create_handle(&g_blas) != rocblas_status_success) return MBEM_EHIP;
+    if (rocblas_set_stream(g_blas, stream) != rocblas_status_success) return MBEM_EHIP;
+    const long KK = (long)(K * K);
+    hipLaunchKernelGGL(k_gram_diag, dim3((unsigned)((K + 255) / 256)), dim3(256), 0, stream, G, (long)K, d);
+    hipLaunchKernelGGL(k_gram_scale, dim3((unsigned)((KK + 255) / 256)), dim3(256), 0, stream, G, (long)K,
+                       d, delta);
+    // G symmetric: its row-major and column-major views are the same matrix
+    if (rocsolver_dpotrf(g_blas, rocblas_fill_lower, (rocblas_int)K, G, (rocblas_int)K, info) !=
+        rocblas_status_success)
+        return MBEM_EHIP;
+    if (rocsolver_dtrtri(g_blas, rocblas_fill_lower, rocblas_diagonal_non_unit, (rocblas_int)K, G,
+                         (rocblas_int)K, info + 1) != rocblas_status_success)
+        return MBEM_EHIP;
+    hipLaunchKernelGGL(k_coarse_tfactors, dim3((unsigned)((KK + 255) / 256)), dim3(256), 0, stream, G,
+                       (long)K, d, Tt, T);
+    return launch_status();
 }
 
 int mbem_bdot(const double* V, int64_t bstride, int64_t cnt, const double* W, int64_t n, int64_t K,

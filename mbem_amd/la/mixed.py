@@ -347,6 +347,7 @@ class MixedReport:
     storage: str = "bf16"
     near_blocks: int = 0
     far_bytes: int = 0
+    coarse: dict | None = None
 
     @property
     def converged(self):
@@ -357,7 +358,9 @@ class MixedReport:
         return self.outer.iterations
 
     def __str__(self):
-        return (f"mixed ({self.storage} far field, FP64 near + outer): outer {self.outer}, "
+        c = "" if not self.coarse else (f", coarse {self.coarse['basis']} rank "
+                                        f"{self.coarse['rank']}")
+        return (f"mixed ({self.storage} far field, FP64 near + outer{c}): outer {self.outer}, "
                 f"inner iterations {self.inner_iterations}")
 
 
@@ -378,10 +381,12 @@ def _side_stream():
 def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
                 inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
                 outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=1500,
-                precond_grouping: str = "region", pattern=None, recycle: int = 90):
+                precond_grouping: str = "region", pattern=None, recycle: int = 90,
+                coarse: str = None):
     """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport).
     ``recycle`` > 0 keeps up to that many recycled vectors across the inner
-    solves (``_RecycledInner``)."""
+    solves (``_RecycledInner``); ``coarse`` ("const", "rigid", "linear")
+    adds the coarse correction of la/coarse.py to the inner block-Jacobi."""
     from .preconditioner import BlockGaussSeidel
     # Two streams (both at the default priority).  The near pattern needs
     # the geometry only: its counting pass is queued on a side stream beside
@@ -400,7 +405,7 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
             near = NearPattern(assembled, k_near)     # counting pass queued, not waited for
     built = {}
 
-    def start_operator(gathered):
+    def start_operator(gathered, chunk_dofs):
         pat = near.result() if near is not None else pattern
         side.wait_event(gathered)
         with torch.cuda.stream(side):
@@ -409,22 +414,37 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     M = BlockGaussSeidel(assembled, max_dense=precond_max_dense, bj_chunk=precond_bj_chunk,
                          grouping=precond_grouping, inverse_dtype="f32", defer_check=True,
                          after_gather=start_operator)
-    t1 = time.perf_counter()
     op = built["op"]
+    cs = None
+    if coarse:
+        # queued after the inversions, so its host work (the groups and
+        # their shape values) runs while the device inverts; on the side
+        # stream behind the operator build: the image of the mixed operator
+        # itself (int8 codes and near blocks; for BF16 / FP32 storage the
+        # FP64 A, whose near blocks the build has put back by then)
+        from .coarse import CoarseSpace
+        with torch.cuda.stream(side):
+            cs = CoarseSpace(assembled, M.chunk_dofs, basis=coarse, op=op)
+    t1 = time.perf_counter()
     main.wait_stream(side)
     M.coupling = op                      # the lower couplings read the mixed operator
     M.finish()                           # one wait for the assembly and both builds
+    P = M
+    if cs is not None:
+        from .coarse import TwoLevel
+        P = TwoLevel(M, cs.finish())
     t2 = time.perf_counter()
     if recycle:
-        inner = _RecycledInner(op, M, inner_rtol, inner_maxiter, op.n, assembled.A_dev.device,
+        inner = _RecycledInner(op, P, inner_rtol, inner_maxiter, op.n, assembled.A_dev.device,
                                kmax=recycle)
     else:
-        inner = _InnerSolve(op, M, inner_rtol, inner_maxiter)
+        inner = _InnerSolve(op, P, inner_rtol, inner_maxiter)
     x, outer = fgmres(assembled, assembled.b_dev, M=inner, rtol=rtol, restart=outer_restart,
                       maxiter=outer_maxiter)
     torch.cuda.current_stream().synchronize()
     t3 = time.perf_counter()
     rep = MixedReport(outer=outer, inner_iterations=inner.iterations, operator_build_s=t2 - t1,
                       precond_build_s=t1 - t0, solve_s=t3 - t2, storage=storage,
-                      near_blocks=op.nnzb, far_bytes=op.far_bytes)
+                      near_blocks=op.nnzb, far_bytes=op.far_bytes,
+                      coarse=(P.summary()["coarse"] if coarse else None))
     return x, rep

@@ -259,12 +259,15 @@ class BlockGaussSeidel:
                      ptr(tmp[q]), int(f32), stream())
             classes.append((cls, mp, tmp))
             c0 = c1 + 1
+        # host copies of the chunk DOF lists (a coarse space is built on them;
+        # ``after_gather`` receives them with the event)
+        self.chunk_dofs = [c[1] for c in chunks]
         self.gathered = torch.cuda.Event()
         self.gathered.record()
         if after_gather is not None:
             # the caller's own launches (e.g. on another stream, after
             # ``gathered``) go ahead of the inversions' long launch sequence
-            after_gather(self.gathered)
+            after_gather(self.gathered, self.chunk_dofs)
         # getrf's info flags, read back once after every class is queued
         # (``finish``), so no class waits on the one before.  The inverse of
         # each class goes to a second buffer (getrf, then A C = I solved with
