@@ -91,16 +91,25 @@ def box_fault_model(n_cells: int = 16, fault_cells=(10, 20), dip: float = 45.0,
     return RegionModel([region]), eps, info
 
 
-def random_box_fault(rng, n_cells_range=(26, 50)):
-    """(model, eps, info) of one random member of the training family: box
-    resolution n_cells even in ``n_cells_range`` (N = 8 n^2 triangles: 5.4k
-    to 20k for the default), fault strike U(0, 180) deg, dip U(20, 90) deg,
-    size U(5, 15) km with cells matched to the box spacing."""
+def random_box_fault_params(rng, n_cells_range=(26, 50)) -> dict:
+    """The draw of ``random_box_fault`` alone (n_cells, strike, dip, size),
+    so a caller can step a generator past geometries without building them."""
     lo, hi = n_cells_range
     n_cells = int(rng.integers(lo // 2, hi // 2 + 1)) * 2
     strike = float(rng.uniform(0.0, 180.0))
     dip = float(rng.uniform(20.0, 90.0))
     size = float(rng.uniform(5.0, 15.0))
+    return {"n_cells": n_cells, "strike": strike, "dip": dip, "size": size}
+
+
+def random_box_fault(rng, n_cells_range=(26, 50), params: dict = None):
+    """(model, eps, info) of one random member of the training family: box
+    resolution n_cells even in ``n_cells_range`` (N = 8 n^2 triangles: 5.4k
+    to 20k for the default), fault strike U(0, 180) deg, dip U(20, 90) deg,
+    size U(5, 15) km with cells matched to the box spacing.  ``params`` (a
+    ``random_box_fault_params`` draw) builds that member instead of drawing."""
+    prm = params if params is not None else random_box_fault_params(rng, n_cells_range)
+    n_cells, strike, dip, size = prm["n_cells"], prm["strike"], prm["dip"], prm["size"]
     h = 40.0 / n_cells
     nf = max(2, int(round(size / h)))
     model, eps, info = box_fault_model(n_cells, fault_cells=(nf, 2 * nf), dip=dip, strike=strike,

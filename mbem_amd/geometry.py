@@ -7,23 +7,27 @@ from __future__ import annotations
 import numpy as np
 
 
-def solid_angle_batch(mesh, points, chunk: int = 4096) -> np.ndarray:
+def solid_angle_batch(mesh, points, chunk: int = 64) -> np.ndarray:
     """(N,) signed solid angle of ``mesh`` at each point: the van Oosterom
     formula 2 atan2(a.(b x c), |a||b||c| + |c| a.b + |a| b.c + |b| a.c) summed
-    over triangles (geometry.py:8-56), chunked over points so the
-    temporaries stay O(chunk x N_tri)."""
+    over triangles (geometry.py:8-56), ``chunk`` points at a time, written
+    out in components over (point, triangle) arrays."""
     pts = np.ascontiguousarray(np.asarray(points, float).reshape(-1, 3))
     tv = np.asarray(mesh.vertices, float)[np.asarray(mesh.triangles)]
+    va, vb, vc = (np.ascontiguousarray(tv[:, k].T) for k in range(3))      # (3, N_tri) each
     out = np.empty(pts.shape[0])
-    step = max(1, int(chunk * 1024 // max(tv.shape[0], 1)))
-    for p0 in range(0, pts.shape[0], step):
-        p = pts[p0:p0 + step, None, :]
-        a, b, c = tv[None, :, 0] - p, tv[None, :, 1] - p, tv[None, :, 2] - p
-        ra, rb, rc = (np.linalg.norm(v, axis=2) for v in (a, b, c))
-        numer = np.einsum("nti,nti->nt", a, np.cross(b, c))
-        denom = (ra * rb * rc + rc * np.einsum("nti,nti->nt", a, b)
-                 + ra * np.einsum("nti,nti->nt", b, c) + rb * np.einsum("nti,nti->nt", a, c))
-        out[p0:p0 + step] = (2.0 * np.arctan2(numer, denom)).sum(axis=1)
+    for p0 in range(0, pts.shape[0], chunk):
+        p = pts[p0:p0 + chunk]
+        ax, ay, az = (va[k][None, :] - p[:, k:k + 1] for k in range(3))
+        bx, by, bz = (vb[k][None, :] - p[:, k:k + 1] for k in range(3))
+        cx, cy, cz = (vc[k][None, :] - p[:, k:k + 1] for k in range(3))
+        ra = np.sqrt(ax * ax + ay * ay + az * az)
+        rb = np.sqrt(bx * bx + by * by + bz * bz)
+        rc = np.sqrt(cx * cx + cy * cy + cz * cz)
+        numer = ax * (by * cz - bz * cy) + ay * (bz * cx - bx * cz) + az * (bx * cy - by * cx)
+        denom = (ra * rb * rc + rc * (ax * bx + ay * by + az * bz)
+                 + ra * (bx * cx + by * cy + bz * cz) + rb * (ax * cx + ay * cy + az * cz))
+        out[p0:p0 + chunk] = (2.0 * np.arctan2(numer, denom)).sum(axis=1)
     return out
 
 
