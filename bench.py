@@ -692,7 +692,7 @@ def main():
           f"{time.perf_counter() - t_setup:.1f} s", flush=True)
 
     def timed(step_fn, label, steps):
-        times, errs, iters = [], [], []
+        times, errs, iters, asm_s = [], [], [], []
         asm_k = x = rep = None
         torch.cuda.synchronize()
         for k in range(steps):
@@ -705,13 +705,17 @@ def main():
             errs.append(float(torch.linalg.vector_norm(x - x_ref)
                               / torch.linalg.vector_norm(x_ref)))
             iters.append(rep.iterations)
+            asm_s.append(asm_k.assembly.seconds)
             ph = ", ".join(f"{k_}={v:.3f}" for k_, v in asm_k.timings.items())
             print(f"[bench] {label} step {k}: {times[-1]:.3f} s ({ph}; {rep}; "
                   f"rel err {errs[-1]:.2e})", flush=True)
         summary = getattr(rep, "precond_summary", None)
-        del asm_k
+        a = asm_k.assembly
+        pairs = {"t_stored": a.t_pairs_stored, "t_rowsum_only": a.t_pairs_rowsum_only,
+                 "u": a.u_pairs, "rhs": a.rhs_pairs}
+        del asm_k, a
         return {"times": times, "errs": errs, "iters": iters, "rep": rep, "x": x,
-                "precond": summary}
+                "precond": summary, "assembly_s": asm_s, "pairs": pairs}
 
     def step_plain():
         a, x = step()
@@ -778,6 +782,26 @@ def main():
         return out
 
     roof_fp64 = roofline("k_gemv", gemv_ms, gemv_bytes, "FGMRES matvec, n x n FP64")
+    # the assembly's executed FP64 rate: the pairs each step's assembly ran x
+    # the executed flops per pair of the block kernels (PMC counters), over
+    # the assembly's device time (HIP events around all of its launches, so
+    # the source preparation, the fault right-hand side and the diagonal
+    # updates count as time but not as flops)
+    ex = _executed_flops()
+    asm_rate = None
+    if ex.get("t_block") and ex.get("u_block"):
+        res = results[main_label]
+        pr = res["pairs"]
+        flops = (ex["t_block"] * (pr["t_stored"] + pr["t_rowsum_only"]) + ex["u_block"] * pr["u"])
+        sec = float(np.median(res["assembly_s"]))
+        asm_rate = {"achieved": flops / sec / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                    "frac": flops / sec / 1e12 / FP64_PEAK_TFLOPS, "assembly_s": sec,
+                    "executed_flops": flops, "pairs": pr,
+                    "executed_flops_per_pair": {"t": ex["t_block"], "u": ex["u_block"]},
+                    "source": "fp64_executed_flops.md (SQ_INSTS_VALU_*_F64 per dispatch)"}
+        print(f"[bench] assembly {sec * 1e3:.1f} ms, {flops:.3e} executed FP64 flops: "
+              f"{asm_rate['achieved']:.1f} TFLOP/s = {100 * asm_rate['frac']:.0f} % of peak",
+              flush=True)
     if main_label == "mixed" and far is not None:
         roof_main = roofline(far[0], far[1], far[2],
                              f"mixed far-field matvec, n x n {args.storage}", ref=far[3:])
@@ -824,6 +848,7 @@ def main():
         },
         "roofline": roof_main,
         "roofline_fp64_matvec": roof_fp64,
+        "roofline_assembly": asm_rate,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
