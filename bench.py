@@ -360,9 +360,7 @@ def bench_laplace(args):
         "roofline": {"bound": "mfma", "achieved": flops / gemm_ms / 1e9,
                      "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                      "frac": flops / gemm_ms / 1e9 / FP64_PEAK_TFLOPS, "traffic": None,
-                     "kernel": ("library DGEMM [A0 | A1][X ; X g] (rocBLAS, FP64 MFMA)"
-                                if op.cfg == op.LIB else
-                                "k_mgemm (A0 X + A1 X diag(g), FP64 MFMA)"),
+                     "kernel": op.describe() + " (FP64 MFMA)",
                      "algorithmic_flops": flops, "avg_ms": gemm_ms},
         "cpu_baseline": None,
     }

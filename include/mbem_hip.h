@@ -341,6 +341,10 @@ int mbem_bj_apply_f32v(const int64_t *wave_table, int64_t nwaves, const int64_t 
  * the count, and {BM, BN, BK, waves along M, waves along N} of one. */
 int mbem_mgemm_configs(void);
 int mbem_mgemm_config(int cfg, int32_t *out5);
+/* 1 if configuration cfg has a form within the LDS: the two-operator form of
+ * mbem_gemm_g (dual != 0) or the one-operator form of mbem_gemm_rm and
+ * mbem_gemm_grouped (dual == 0); 0 otherwise. */
+int mbem_mgemm_config_fits(int cfg, int dual);
 
 /* Y = alpha (A0 X + A1 X diag(g)) + beta Y: A0, A1 row-major m x kd (even
  * lda, 16-byte aligned), X kd x ncols (even ldx, aligned), g (ncols).
@@ -348,6 +352,14 @@ int mbem_mgemm_config(int cfg, int32_t *out5);
 int mbem_gemm_g(const double *A0, const double *A1, int64_t lda, int64_t m, int64_t kd,
                 const double *X, int64_t ldx, int64_t ncols, const double *g, double *Y,
                 int64_t ldy, double alpha, double beta, int cfg, hipStream_t stream);
+
+/* Row-major Y (m x ncols) = alpha A (m x kd) X (kd x ncols) + beta Y on the
+ * project's FP64 matrix-core kernel, tile configuration cfg (even lda and
+ * ldx, 16-byte aligned A and X): the batched operator as one GEMM on
+ * [A0 | A1] [X ; X diag(g)], the same product as mbem_dgemm_rowmajor. */
+int mbem_gemm_rm(const double *A, int64_t lda, int64_t m, int64_t kd, const double *X, int64_t ldx,
+                 int64_t ncols, double *Y, int64_t ldy, double alpha, double beta, int cfg,
+                 hipStream_t stream);
 
 /* Row-major C (m x n) = alpha A (m x k) B (k x n) + beta C through the
  * library DGEMM (rocBLAS, matrix cores, 64-bit sizes): the autotuner's library

@@ -86,6 +86,8 @@ _SIGS = {
     # batched Laplace-state solves
     "mbem_mgemm_configs": [],
     "mbem_mgemm_config": [_int, _p],
+    "mbem_mgemm_config_fits": [_int, _int],
+    "mbem_gemm_rm": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p, _i64, _d, _d, _int, _p],
     "mbem_gemm_g": [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _int, _p],
     "mbem_work_item_bytes": [],
     "mbem_dgemm_rowmajor": [_i64, _i64, _i64, _d, _p, _i64, _p, _i64, _d, _p, _i64, _p],

@@ -451,16 +451,29 @@ constexpr Cfg kCfgs[] = {
     {64, 128, 8, 2, 4},
     {128, 64, 8, 4, 2},
     {128, 128, 16, 2, 2},
+    {256, 128, 8, 4, 2},
 };
 constexpr int kNCfg = sizeof(kCfgs) / sizeof(kCfgs[0]);
 
-template <int BM, int BN, int BK, int WM, int WN, bool DUAL, bool TABLE>
-void launch_cfg(dim3 grid, hipStream_t st, const double* A0, const double* A1, long lda, long M,
-                long Kd, const double* X, long ldx, long N, const double* g, double* Y, long ldy,
-                double alpha, double beta, const Work* work)
+// static LDS of a k_mgemm instance (double-buffered tiles, rows padded by 2)
+constexpr long mgemm_lds(int BM, int BN, int BK, bool DUAL)
 {
-    hipLaunchKernelGGL((k_mgemm<BM, BN, BK, WM, WN, DUAL, TABLE>), grid, dim3(WM * WN * 64), 0, st,
-                       A0, A1, lda, M, Kd, X, ldx, N, g, Y, ldy, alpha, beta, work);
+    return 8L * 2 * BK * ((BM + 2) * (DUAL ? 2 : 1) + (BN + 2) * (DUAL ? 2 : 1));
+}
+constexpr long kLdsBytes = 160L * 1024;
+
+template <int BM, int BN, int BK, int WM, int WN, bool DUAL, bool TABLE>
+int launch_cfg(dim3 grid, hipStream_t st, const double* A0, const double* A1, long lda, long M,
+               long Kd, const double* X, long ldx, long N, const double* g, double* Y, long ldy,
+               double alpha, double beta, const Work* work)
+{
+    if constexpr (mgemm_lds(BM, BN, BK, DUAL) > kLdsBytes) {
+        return MBEM_EINVAL;        // this tile does not fit the LDS in this form
+    } else {
+        hipLaunchKernelGGL((k_mgemm<BM, BN, BK, WM, WN, DUAL, TABLE>), grid, dim3(WM * WN * 64), 0,
+                           st, A0, A1, lda, M, Kd, X, ldx, N, g, Y, ldy, alpha, beta, work);
+        return launch_status();
+    }
 }
 
 template <bool DUAL, bool TABLE>
@@ -469,7 +482,7 @@ int dispatch(int cfg, dim3 grid, hipStream_t st, const double* A0, const double*
              long ldy, double alpha, double beta, const Work* work)
 {
 #define MBEM_CFG(i, BM, BN, BK, WM, WN) \
-    case i: launch_cfg<BM, BN, BK, WM, WN, DUAL, TABLE>(grid, st, A0, A1, lda, M, Kd, X, ldx, N, g, Y, ldy, alpha, beta, work); break;
+    case i: return launch_cfg<BM, BN, BK, WM, WN, DUAL, TABLE>(grid, st, A0, A1, lda, M, Kd, X, ldx, N, g, Y, ldy, alpha, beta, work);
     switch (cfg) {
         MBEM_CFG(0, 64, 64, 8, 2, 2)
         MBEM_CFG(1, 64, 128, 8, 2, 2)
@@ -479,10 +492,18 @@ int dispatch(int cfg, dim3 grid, hipStream_t st, const double* A0, const double*
         MBEM_CFG(5, 64, 128, 8, 2, 4)
         MBEM_CFG(6, 128, 64, 8, 4, 2)
         MBEM_CFG(7, 128, 128, 16, 2, 2)
+        MBEM_CFG(8, 256, 128, 8, 4, 2)
         default: return MBEM_EINVAL;
     }
 #undef MBEM_CFG
-    return launch_status();
+}
+
+// whether configuration cfg has a DUAL (two-operator) form within the LDS
+template <bool DUAL>
+bool cfg_fits(int cfg)
+{
+    const Cfg c = kCfgs[cfg];
+    return mgemm_lds(c.bm, c.bn, c.bk, DUAL) <= kLdsBytes;
 }
 
 inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 15) == 0; }
@@ -557,6 +578,27 @@ int mbem_mgemm_config(int cfg, int32_t* out5)
     out5[3] = kCfgs[cfg].wm;
     out5[4] = kCfgs[cfg].wn;
     return MBEM_OK;
+}
+
+int mbem_mgemm_config_fits(int cfg, int dual)
+{
+    if (cfg < 0 || cfg >= kNCfg) return 0;
+    return dual ? cfg_fits<true>(cfg) : cfg_fits<false>(cfg);
+}
+
+int mbem_gemm_rm(const double* A, int64_t lda, int64_t m, int64_t kd, const double* X, int64_t ldx,
+                 int64_t ncols, double* Y, int64_t ldy, double alpha, double beta, int cfg,
+                 hipStream_t stream)
+{
+    if (m < 0 || kd < 0 || ncols < 0 || cfg < 0 || cfg >= kNCfg) return MBEM_EINVAL;
+    if (m == 0 || ncols == 0) return MBEM_OK;
+    if (!A || !X || !Y || lda < kd || ldx < ncols || ldy < ncols) return MBEM_EINVAL;
+    if ((lda & 1) || (ldx & 1) || !aligned16(A) || !aligned16(X)) return MBEM_EINVAL;
+    const Cfg c = kCfgs[cfg];
+    const long gx = (m + c.bm - 1) / c.bm, gy = (ncols + c.bn - 1) / c.bn;
+    if (gx > INT32_MAX || gy > 65535) return MBEM_EINVAL;
+    return dispatch<false, false>(cfg, dim3((unsigned)gx, (unsigned)gy), stream, A, nullptr, lda, m,
+                                  kd, X, ldx, ncols, nullptr, Y, ldy, alpha, beta, nullptr);
 }
 
 int mbem_gemm_g(const double* A0, const double* A1, int64_t lda, int64_t m, int64_t kd,

@@ -43,11 +43,13 @@ I64 = torch.int64
 class GOperator:
     """Y = alpha B(g) X + beta Y for n x K blocks, B(g_c) = A0 + g_c A1.
 
-    Candidates: the compiled tiles of the project's matrix-core kernel
-    (``mbem_gemm_g``) and, when A0 and A1 sit side by side in one n x 2L
-    row-major buffer (``pair=(AA, L)``: A0 = AA[:, :n], A1 = AA[:, L:L+n]),
-    the library DGEMM on [A0 | A1] [X ; X diag(g)] (``LIB``).  ``tune``
-    times them on the real shape and keeps the fastest."""
+    Candidates: the compiled tiles of the project's matrix-core kernel in
+    its two-operator form (``mbem_gemm_g``, ids 0..C-1) and, when A0 and A1
+    sit side by side in one n x 2L row-major buffer (``pair=(AA, L)``:
+    A0 = AA[:, :n], A1 = AA[:, L:L+n]), the one GEMM [A0 | A1] [X ; X diag(g)]
+    through the library DGEMM (``LIB`` = C) or through the project's kernel
+    in its one-operator form (``mbem_gemm_rm``, ids C+1+cfg).  ``tune`` times
+    them on the real shape and keeps the fastest."""
 
     def __init__(self, A0: torch.Tensor, A1: torch.Tensor, g: torch.Tensor, pair=None):
         n = A0.shape[0]
@@ -69,8 +71,23 @@ class GOperator:
                 raise ValueError("pair must be the n x 2L buffer holding A0 and A1")
 
     def candidates(self):
-        c = [i for i, _ in autotune.configs()]
-        return c + [self.LIB] if self.pair is not None else c
+        lib = _lib.load()
+        cfgs = [i for i, _ in autotune.configs()]
+        c = [i for i in cfgs if lib.mbem_mgemm_config_fits(i, 1)]
+        if self.pair is None:
+            return c
+        return c + [self.LIB] + [self.LIB + 1 + i for i in cfgs if lib.mbem_mgemm_config_fits(i, 0)]
+
+    def describe(self, cfg=None) -> str:
+        """What candidate ``cfg`` (default: the chosen one) is."""
+        cfg = self.cfg if cfg is None else cfg
+        tiles = dict(autotune.configs())
+        if cfg == self.LIB:
+            return "library DGEMM on [A0 | A1] [X ; X g]"
+        t = tiles[cfg - self.LIB - 1] if cfg > self.LIB else tiles[cfg]
+        form = "one GEMM on [A0 | A1] [X ; X g]" if cfg > self.LIB else "two-operator form"
+        return (f"project kernel {t['bm']}x{t['bn']}x{t['bk']} tile, {t['wm']}x{t['wn']} waves, "
+                f"{form}")
 
     def tune(self, X, Y):
         """Pick the configuration for this shape (times every candidate)."""
@@ -87,13 +104,17 @@ class GOperator:
             self.X2 = torch.zeros((2 * L, self.K), dtype=F64, device=self.A0.device)
 
     def _launch(self, X, Y, alpha, beta, cfg):
-        if cfg == self.LIB:
+        if cfg >= self.LIB:
             AA, L = self.pair
             n, K = self.n, self.K
             call("mbem_bcopy", ptr(X), ptr(self.X2), n * K, stream())
             call("mbem_bscale", ptr(X), ptr(self.g), ptr(self.X2[L:]), n, K, stream())
-            call("mbem_dgemm_rowmajor", n, K, 2 * L, float(alpha), ptr(AA), AA.stride(0),
-                 ptr(self.X2), K, float(beta), ptr(Y), Y.stride(0), stream())
+            if cfg == self.LIB:
+                call("mbem_dgemm_rowmajor", n, K, 2 * L, float(alpha), ptr(AA), AA.stride(0),
+                     ptr(self.X2), K, float(beta), ptr(Y), Y.stride(0), stream())
+            else:
+                call("mbem_gemm_rm", ptr(AA), AA.stride(0), n, 2 * L, ptr(self.X2), K, K, ptr(Y),
+                     Y.stride(0), float(alpha), float(beta), int(cfg - self.LIB - 1), stream())
             return
         call("mbem_gemm_g", ptr(self.A0), ptr(self.A1), self.A0.stride(0), self.n, self.n, ptr(X),
              X.stride(0), self.K, ptr(self.g), ptr(Y), Y.stride(0), float(alpha), float(beta),

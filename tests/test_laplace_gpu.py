@@ -54,6 +54,7 @@ def _gemm_ref(A0, A1, X, g):
 
 
 def test_gemm_g_every_config():
+    from mbem_amd import _lib
     from mbem_amd._lib import call, ptr, stream
     from mbem_amd.la import autotune
     torch.manual_seed(1)
@@ -65,7 +66,14 @@ def test_gemm_g_every_config():
     g = torch.rand(K, dtype=torch.float64, device=dev) + 1.0
     Y0 = torch.randn(m, K, dtype=torch.float64, device=dev)
     ref = _gemm_ref(A0, A1, X, g)
+    lib = _lib.load()
     for cfg, c in autotune.configs():
+        if not lib.mbem_mgemm_config_fits(cfg, 1):
+            # a tile whose two-operator form exceeds the LDS is refused
+            with pytest.raises(_lib.MbemError):
+                call("mbem_gemm_g", ptr(A0), ptr(A1), ld, m, m, ptr(X), K, K, ptr(g), ptr(X), K,
+                     1.0, 0.0, cfg, stream())
+            continue
         print(f"gemm_g cfg {cfg} {c}", flush=True)
         Y = torch.empty(m, K, dtype=torch.float64, device=dev)
         call("mbem_gemm_g", ptr(A0), ptr(A1), ld, m, m, ptr(X), K, K, ptr(g), ptr(Y), K, 1.0, 0.0,
@@ -133,7 +141,10 @@ def test_library_candidate_matches_kernel():
     op._prepare()
     Y0 = torch.randn(n, K, dtype=torch.float64, device=dev)
     ref = -0.5 * _gemm_ref(A0, A1, X, g) + 2.0 * Y0
-    for cfg in (0, op.LIB):
+    cands = op.candidates()
+    assert 0 in cands and op.LIB in cands and op.LIB + 1 in cands
+    for cfg in cands:
+        print(f"operator candidate {cfg}: {op.describe(cfg)}", flush=True)
         op.cfg = cfg
         Y = Y0.clone()
         op.apply(X, Y, alpha=-0.5, beta=2.0)
