@@ -102,11 +102,35 @@ def _wsum(w, vals):
     return out.reshape(vals.shape[1:])
 
 
-def _integrate(kind, tv, obs, eps, mu, lam, normal, depth, n):
+def lagrange_weight(tv, order, k):
+    """y (q, 3) -> N_k(y): the textbook Lagrange basis of the triangle tv in
+    barycentric coordinates (P1: lam_k; P2: vertices lam_i (2 lam_i - 1),
+    edge midpoints 4 lam_i lam_j on edges 12, 23, 31), written independently
+    of the node-lattice product formula of the kernels under test."""
+    tv = np.asarray(tv, float)
+    T = np.stack([tv[1] - tv[0], tv[2] - tv[0]], axis=1)      # (3, 2)
+
+    def N(y):
+        st = np.linalg.lstsq(T, (y - tv[0]).T, rcond=None)[0].T   # (q, 2)
+        lam = np.column_stack([1.0 - st[:, 0] - st[:, 1], st[:, 0], st[:, 1]])
+        if order == 0:
+            return np.ones(y.shape[0])
+        if order == 1:
+            return lam[:, k]
+        if k < 3:
+            return lam[:, k] * (2.0 * lam[:, k] - 1.0)
+        i, j = ((0, 1), (1, 2), (2, 0))[k - 3]
+        return 4.0 * lam[:, i] * lam[:, j]
+    return N
+
+
+def _integrate(kind, tv, obs, eps, mu, lam, normal, depth, n, weight=None):
     nu = lam / (2.0 * (lam + mu))
     C1 = 1.0 / (16.0 * np.pi * mu * (1.0 - nu))
     fG, fdG, fd2G = _symbolic()
     y, w = _points(tv, depth, n)
+    if weight is not None:
+        w = w * weight(y)
     d = np.asarray(obs, float)[None, :] - y
     nrm = np.asarray(normal, float)
     if kind == "G":
@@ -144,17 +168,18 @@ def _integrate(kind, tv, obs, eps, mu, lam, normal, depth, n):
     raise ValueError(kind)
 
 
-def pair(kind, tv, obs, eps, mu=1.0, lam=1.0, normal=None, depth=3, n=16):
+def pair(kind, tv, obs, eps, mu=1.0, lam=1.0, normal=None, depth=3, n=16, weight=None):
     """Quadrature of one pair at ``depth`` and ``depth + 1`` subdivisions.
 
     kind: "G" (3,3) force->displacement; "T" (3,3) slip->displacement with
     the source normal; "S" (3,3,3) force->stress; "H" (3,3,3) slip->total
     stress; "E" (3,3,3) eigenstress; "I" (4,) scalar moments I1..I7.
+    ``weight(y)``, if given, multiplies the integrand (a nodal shape function).
     """
     tv = np.asarray(tv, float)
     if normal is None:
         c = np.cross(tv[1] - tv[0], tv[2] - tv[0])
         normal = c / np.linalg.norm(c)
-    a = _integrate(kind, tv, obs, eps, mu, lam, normal, depth, n)
-    b = _integrate(kind, tv, obs, eps, mu, lam, normal, depth + 1, n)
+    a = _integrate(kind, tv, obs, eps, mu, lam, normal, depth, n, weight)
+    b = _integrate(kind, tv, obs, eps, mu, lam, normal, depth + 1, n, weight)
     return a, b
