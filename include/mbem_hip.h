@@ -337,6 +337,52 @@ int mbem_bj_apply_f32v(const int64_t *wave_table, int64_t nwaves, const int64_t 
  * B(g_c) y_c = b_c.  None of them replaces a reference function one-for-one;
  * together they stand in for that state loop. */
 
+/* ---- Lagrange P0/P1/P2 nodal densities (csrc/nodal.hip) -----------------
+ * Replace tri_nodal.py's numba drivers: the matrices and basis stacks of
+ * u_basis_matrices / t_basis_matrices / u_matrix_direct / t_matrix_direct
+ * (tri_nodal.py:1257-1306, via _u/_t_basis_nodal and _u/_t_direct_nodal
+ * :958-1043), the contractions u_disp_contract / t_disp_contract /
+ * kelvin_stress_contract / dd_stress_contract / eigenstress_contract
+ * (:1309-1363, via :1055-1172) and traction_matrix (:1370-1404, via
+ * _traction_matrix_nodal :1175-1227).  Device pointers; columns
+ * 3 (K s + k) + j; densities (K ns, 3).
+ *
+ * The spec of a launch: ispec (host, 18 int64) = {order, K, q_split,
+ * n_terms, nq, table degree for n = -1,1,3,5,7, edge degree for
+ * m = -1,1,3,5, kernel degree for n = 1,3,5,7} (-1 = none), fspec (host, 4)
+ * = {D*, D*_distant, small-|u| / rho, series |u| / rho}, tables (device,
+ * mbem_nodal_table_doubles(order, n_terms, nq) doubles) = series
+ * coefficients C(-m/2, j) (4 x n_terms) | node lattice (K x 3) | Gauss
+ * barycentric points (nq x 3) | weights (nq) | shape values (nq x K). */
+int64_t mbem_nodal_geo_doubles(int64_t ns);
+int64_t mbem_nodal_table_doubles(int64_t order, int64_t n_terms, int64_t nq);
+/* nodal frames of the sources (tri_nodal.py:418-470): geo[30 ns] */
+int mbem_nodal_frames(const double *tri_verts, int64_t ns, double *geo, hipStream_t stream);
+/* kind 0 U, 1 T; coeffs (host: 3 u_coeffs or 6 t_coeffs) gives the direct
+ * matrix out (3 nf x ld), coeffs NULL the basis stack: block b at
+ * out + b bstride (3 or 6 blocks). */
+int mbem_nodal_matrix(int kind, const double *x_field, int64_t nf, const double *tri_verts,
+                      const double *normals, const double *eps, int64_t ns, const double *geo,
+                      const int64_t *ispec, const double *fspec, const double *tables,
+                      const double *coeffs, double *out, int64_t ld, int64_t bstride,
+                      hipStream_t stream);
+/* kind 0 U / 1 T displacement (out nf x 3, coeffs on the host), 2 force ->
+ * stress, 3 slip -> total stress, 4 eigenstress (out nf x 9); (mu, nu, lam)
+ * for the stresses. */
+int mbem_nodal_contract(int kind, const double *x_field, int64_t nf, const double *tri_verts,
+                        const double *normals, const double *eps, int64_t ns, const double *geo,
+                        const int64_t *ispec, const double *fspec, const double *tables,
+                        const double *density, const double *coeffs, double mu, double nu,
+                        double lam, double *out, hipStream_t stream);
+/* traction rows accumulated into out (3 nf x ld, ld >= 3 n_cols): which 0
+ * slip -> elastic, 1 force, 2 eigenstress, 3 slip -> total; ids (device,
+ * ns x K) must lie in [0, n_cols) (checked by the caller). */
+int mbem_nodal_traction(int which, const double *x_field, const double *n_field, int64_t nf,
+                        const double *tri_verts, const double *normals, const double *eps,
+                        int64_t ns, const double *geo, const int64_t *ispec, const double *fspec,
+                        const double *tables, const int64_t *ids, int64_t n_cols, double mu,
+                        double nu, double lam, double *out, int64_t ld, hipStream_t stream);
+
 /* Tile configurations of the FP64 MFMA GEMM (the autotuner's candidates):
  * the count, and {BM, BN, BK, waves along M, waves along N} of one. */
 int mbem_mgemm_configs(void);

@@ -87,6 +87,14 @@ _SIGS = {
     "mbem_mgemm_configs": [],
     "mbem_mgemm_config": [_int, _p],
     "mbem_mgemm_config_fits": [_int, _int],
+    "mbem_nodal_geo_doubles": [_i64],
+    "mbem_nodal_table_doubles": [_i64, _i64, _i64],
+    "mbem_nodal_frames": [_p, _i64, _p, _p],
+    "mbem_nodal_matrix": [_int, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _i64, _i64, _p],
+    "mbem_nodal_contract": [_int, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _p, _d, _d, _d,
+                            _p, _p],
+    "mbem_nodal_traction": [_int, _p, _p, _i64, _p, _p, _p, _i64, _p, _p, _p, _p, _p, _i64, _d, _d,
+                            _d, _p, _i64, _p],
     "mbem_gemm_rm": [_p, _i64, _i64, _i64, _p, _i64, _i64, _p, _i64, _d, _d, _int, _p],
     "mbem_gemm_g": [_p, _p, _i64, _i64, _i64, _p, _i64, _i64, _p, _p, _i64, _d, _d, _int, _p],
     "mbem_work_item_bytes": [],
@@ -137,7 +145,8 @@ _SIGS = {
     "mbem_lr_outer": [_p, _i64, _p, _i64, _i64, _i64, _i64, _p, _p],
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }
-_RESTYPE = {"mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
+_RESTYPE = {"mbem_nodal_geo_doubles": _i64, "mbem_nodal_table_doubles": _i64,
+            "mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
 
 _lib = None
 

@@ -27,6 +27,11 @@ HIPCC_FLAGS = [
 ]
 
 
+# per-source extra flags: the nodal kernels' cancellation-sensitive
+# differences round as written (no fused multiply-add contraction)
+FILE_FLAGS = {"nodal.hip": ["-ffp-contract=off"]}
+
+
 def hipcc() -> str:
     exe = shutil.which("hipcc") or os.path.join(ROCM, "bin", "hipcc")
     if not os.path.exists(exe):
@@ -39,7 +44,8 @@ def build(verbose: bool = False) -> str:
     objs = []
     for src in sorted(glob.glob(os.path.join(CSRC, "*.hip"))):
         obj = os.path.join(BUILD_DIR, os.path.basename(src)[:-4] + ".o")
-        cmd = [hipcc(), *HIPCC_FLAGS, "-c", src, "-o", obj]
+        cmd = [hipcc(), *HIPCC_FLAGS, *FILE_FLAGS.get(os.path.basename(src), []), "-c", src,
+               "-o", obj]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)

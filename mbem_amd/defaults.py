@@ -36,3 +36,11 @@ PRONY_RCOND = 1.0e-13              # defaults.py:1067
 LAPLACE_TAIL_DECADES = 12.0        # defaults.py:1062
 EXTENDED_BURGERS_PER_DECADE = 4    # defaults.py:1082
 STEPPER_TAIL_DECADES = 1.0         # defaults.py:1117
+NODAL_D_STAR = 10.0                # defaults.py:916
+NODAL_FAR_GAUSS_N = 12             # defaults.py:917
+NODAL_FAR_GAUSS_N_DISTANT = 8      # defaults.py:918
+NODAL_D_STAR_DISTANT = 40.0        # defaults.py:919
+NODAL_SERIES_U_OVER_RHO = 2.0      # defaults.py:925
+NODAL_SERIES_TERMS = 32            # defaults.py:926
+NODAL_SMALL_U_OVER_RHO = 0.5       # defaults.py:927
+NODAL_MIN_HEIGHT_OVER_L = 1e-3     # defaults.py:932

@@ -228,62 +228,89 @@ class TBasis:
         return self.stack.nbytes
 
 
-def _order0(order):
-    if order != 0:
-        raise NotImplementedError("only P0 (order 0) densities are built on the GPU path")
+def _condensed(kernel_call, x_field, tri_verts, normals, eps_arr, order, ids, n_cols):
+    """``kernel_call`` on chunks of ASSEMBLY_SOURCE_CHUNK source elements, each
+    chunk's element-major columns summed into the shared columns before the
+    next (basis.py:354-369)."""
+    ids = np.ascontiguousarray(np.asarray(ids, dtype=np.int64))
+    from .tri_nodal import n_nodes
+    N_s = tri_verts.shape[0]
+    if ids.shape != (N_s, n_nodes(order)):
+        raise ValueError(f"ids shape {ids.shape} != ({N_s}, {n_nodes(order)})")
+    out = np.zeros((3 * x_field.shape[0], 3 * int(n_cols)))
+    step = int(defaults.ASSEMBLY_SOURCE_CHUNK)
+    for c0 in range(0, N_s, step):
+        c1 = min(N_s, c0 + step)
+        blk = kernel_call(x_field, tri_verts[c0:c1], None if normals is None else normals[c0:c1],
+                          eps_arr[c0:c1], order)
+        condense_columns(blk, ids[c0:c1], n_cols, out=out)
+    return out
 
 
 def assemble_u_basis(mesh_field, mesh_source, eps, arrays=None, order: int = 0) -> UBasis:
-    """basis.py:336-342 (GPU kernel)."""
-    from . import tri_kernels as tk
-    _order0(order)
+    """basis.py:336-342 (GPU kernels; P0 pair code at order 0, nodal beyond)."""
+    from . import tri_nodal as tn
     x_field = _field_points(mesh_field)
     tri_verts, _ = _source_arrays(mesh_source)
-    return UBasis(tk.u_basis_matrices(x_field, tri_verts, as_eps_array(eps, tri_verts.shape[0])))
+    return UBasis(tn.u_basis_matrices(x_field, tri_verts, as_eps_array(eps, tri_verts.shape[0]),
+                                      order))
 
 
 def assemble_t_basis(mesh_field, mesh_source, eps, arrays=None, order: int = 0) -> TBasis:
-    """basis.py:345-351 (GPU kernel)."""
-    from . import tri_kernels as tk
-    _order0(order)
+    """basis.py:345-351 (GPU kernels; P0 pair code at order 0, nodal beyond)."""
+    from . import tri_nodal as tn
     x_field = _field_points(mesh_field)
     tri_verts, normals = _source_arrays(mesh_source)
-    return TBasis(tk.t_basis_matrices(x_field, tri_verts, normals,
-                                      as_eps_array(eps, tri_verts.shape[0])))
+    return TBasis(tn.t_basis_matrices(x_field, tri_verts, normals,
+                                      as_eps_array(eps, tri_verts.shape[0]), order))
 
 
 def assemble_u_matrix(mesh_field, mesh_source, material, eps, arrays=None, order: int = 0,
                       ids=None, n_cols=None) -> np.ndarray:
-    """One-shot U matrix (3N_f, 3N_s) (basis.py:372-386; P0, no shared ids)."""
-    from . import tri_kernels as tk
-    _order0(order)
-    if ids is not None:
-        raise NotImplementedError("shared-node columns (continuous layouts) are not in mbem_amd")
+    """One-shot U matrix (3 N_f, 3 K N_s), or (3 N_f, 3 n_cols) condensed onto
+    shared-node columns when ``ids`` (N_s, K) is given (basis.py:372-386)."""
+    from . import tri_nodal as tn
     x_field = _field_points(mesh_field)
     tri_verts, _ = _source_arrays(mesh_source)
+    eps_arr = as_eps_array(eps, tri_verts.shape[0])
     g1, g2, g3 = u_coeffs(material.mu, material.lam)
-    return tk.u_matrix_direct(x_field, tri_verts, as_eps_array(eps, tri_verts.shape[0]),
-                              g1, g2, g3)
+    if ids is None:
+        return tn.u_matrix_direct(x_field, tri_verts, eps_arr, g1, g2, g3, order)
+    return _condensed(lambda x, tv, _n, e, p: tn.u_matrix_direct(x, tv, e, g1, g2, g3, p),
+                      x_field, tri_verts, None, eps_arr, order, ids, n_cols)
 
 
 def assemble_t_matrix(mesh_field, mesh_source, material, eps, arrays=None, order: int = 0,
                       ids=None, n_cols=None) -> np.ndarray:
-    """One-shot T matrix (3N_f, 3N_s) (basis.py:389-403; P0, no shared ids)."""
-    from . import tri_kernels as tk
-    _order0(order)
-    if ids is not None:
-        raise NotImplementedError("shared-node columns (continuous layouts) are not in mbem_amd")
+    """One-shot T matrix (3 N_f, 3 K N_s), or condensed onto shared-node
+    columns when ``ids`` is given (basis.py:389-403)."""
+    from . import tri_nodal as tn
     x_field = _field_points(mesh_field)
     tri_verts, normals = _source_arrays(mesh_source)
+    eps_arr = as_eps_array(eps, tri_verts.shape[0])
     c = t_coeffs(material.mu, material.lam)
-    return tk.t_matrix_direct(x_field, tri_verts, normals,
-                              as_eps_array(eps, tri_verts.shape[0]), *c)
+    if ids is None:
+        return tn.t_matrix_direct(x_field, tri_verts, normals, eps_arr, *c, order)
+    return _condensed(lambda x, tv, n, e, p: tn.t_matrix_direct(x, tv, n, e, *c, p),
+                      x_field, tri_verts, normals, eps_arr, order, ids, n_cols)
 
 
 def assemble_traction_matrix(x_field, n_field, mesh_source, material, eps, arrays=None,
                              order: int = 0, kind: str = "slip", ids=None, n_cols=None):
-    """Traction rows of a nodal density (basis.py:406-430): the rows of solved
-    (FAULT_TRACTION) faults, not in mbem_amd."""
-    raise NotImplementedError("assemble_traction_matrix: the traction rows of solved "
-                              "(FAULT_TRACTION) faults are not in mbem_amd")
-
+    """Traction rows (3 N_f, 3 n_cols) at the points ``x_field`` with normals
+    ``n_field`` of a nodal density on ``mesh_source`` (basis.py:406-427):
+    ``kind="slip"`` the ELASTIC traction of a displacement jump (eigenstress
+    removed: the rows of a traction-BC fault), ``"force"`` the Kelvin single
+    layer's, ``"eigen"`` / ``"total"`` the eigenstress alone / left in.
+    Element-major columns unless ``ids`` (N_s, K) maps them onto ``n_cols``
+    shared nodes; the scatter happens inside the kernel."""
+    from . import tri_nodal as tn
+    tri_verts, normals = _source_arrays(mesh_source)
+    eps_arr = as_eps_array(eps, tri_verts.shape[0])
+    if ids is None:
+        ids = identity_ids(order, tri_verts.shape[0])
+        n_cols = ids.size
+    return tn.traction_matrix(np.ascontiguousarray(np.asarray(x_field, float)),
+                              np.ascontiguousarray(np.asarray(n_field, float)),
+                              tri_verts, normals, eps_arr, ids, int(n_cols),
+                              material.mu, material.lam, order, kind)
