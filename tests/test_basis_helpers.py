@@ -67,5 +67,10 @@ def test_mesh_arrays_memo():
     assert v.shape == (1, 3, 3) and memo.source_arrays(m)[0] is v
     pts = np.zeros((2, 3))
     assert np.array_equal(memo.field_points(pts), pts)
-    with pytest.raises(NotImplementedError):
-        kb.assemble_traction_matrix(pts, pts, m, None, 0.1)
+    # the traction rows' kind and ids table are checked before any device call
+    from mbem_amd.mesh import ElasticMaterial
+    mat = ElasticMaterial(mu=1.0, lam=1.0)
+    with pytest.raises(KeyError):
+        kb.assemble_traction_matrix(pts, pts, m, mat, 0.1, kind="bogus")
+    with pytest.raises(ValueError, match="ids shape"):
+        kb.assemble_traction_matrix(pts, pts, m, mat, 0.1, order=1, ids=np.zeros((1, 1)), n_cols=1)
