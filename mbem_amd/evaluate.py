@@ -96,15 +96,16 @@ class _Ctx:
 
 def evaluate_displacement(model: RegionModel, region, solution: dict, points, eps,
                           warn_near: bool = True, near=None) -> np.ndarray:
-    """Displacement (N,3) at interior ``points`` of ``region`` (evaluate.py:194-253)."""
-    if near is not None:
-        raise NotImplementedError("near-face recovery (near=) is not in mbem_amd")
+    """Displacement (N,3) at interior ``points`` of ``region`` (evaluate.py:194-253).
+    ``near`` (a ``recover.NearField`` of this model) reads the pairs close to
+    a boundary patch from the recovered continuous displacement at the
+    narrowed mollification."""
     region = _region(model, region)
     if region.faults:
         from .selfcheck import ensure_fault_convention
         ensure_fault_convention()
     points = np.ascontiguousarray(np.asarray(points, dtype=float).reshape(-1, 3))
-    if warn_near:
+    if warn_near and near is None:
         _warn_near_boundary(points, region)
     ctx = _Ctx(points, eps)
     out = torch.zeros((points.shape[0], 3), dtype=F64, device=ctx.dev)
@@ -121,6 +122,11 @@ def evaluate_displacement(model: RegionModel, region, solution: dict, points, ep
             t_p = _density(model, p, "t", solution)
             if np.any(t_p):
                 dv.u_disp(ctx.x, src, ctx.dens(t_p), g, out, scale=sigma, beta=1.0)
+            if near is not None and near.recovers(p):
+                dbl, sgl, _ = _near_field(points, p, u_p, near.recover(p, u_p), t_p, mat.mu,
+                                          mat.lam, kb.resolve_patch_eps(eps, p), near,
+                                          stress=False)
+                out += sigma * (sgl - dbl)
         for f in region.faults:
             sigma = float(model.orientation(region, f))
             slip = _density(model, f, "slip", solution)
@@ -178,6 +184,87 @@ def near_list(points, cen, radius):
     return chunks, pt_ptr, C[order]
 
 
+def _near_blocks(points, cen, radius):
+    """(P, S) per source chunk with observation points in reach: the points
+    within the one ball covering the chunk members' balls (evaluate.py:278-295)."""
+    from scipy.spatial import cKDTree
+    if points.shape[0] == 0 or cen.shape[0] == 0:
+        return
+    tree = cKDTree(points)
+    for S in _source_chunks(cen, radius):
+        c0 = cen[S].mean(axis=0)
+        reach = float(np.max(np.linalg.norm(cen[S] - c0, axis=1) + radius[S]))
+        P = np.asarray(tree.query_ball_point(c0, reach), dtype=np.intp)
+        if P.size:
+            yield P, S
+
+
+def _near_field(points, patch, u0, u2, t0, mu, lam, eps_arr, near, stress, eigen=True):
+    """What reading a patch's NEAR pairs from the recovered density adds to
+    the element-constant readout: (double, single, eigen), unsigned, on the
+    device (evaluate.py:410-472).  For the elements within near.radius_h of
+    their own h of a point: the double layer of the recovered P2 displacement
+    u2 at the narrowed width in place of the element values u0 at the
+    solve's; the single layer of the same element traction t0 at the
+    narrowed width in place of the solve's; for stress the eigenstress of
+    each of the two double layers."""
+    from .kernels import nodal as nd
+    dev = _lib.device()
+    tv, nrm = kb._source_arrays(patch.mesh)
+    eps_arr = np.ascontiguousarray(np.asarray(eps_arr, float))
+    eps_near = near.eps_factor * eps_arr
+    radius = near.radius_h * kb.element_sizes(patch.mesh)
+    cen = tv.mean(axis=1)
+    n = points.shape[0]
+    shape = (n, 3, 3) if stress else (n, 3)
+    dbl = torch.zeros(shape, dtype=F64, device=dev)
+    sgl = torch.zeros(shape, dtype=F64, device=dev)
+    eig = torch.zeros(shape, dtype=F64, device=dev) if stress else None
+    do_u, do_t = bool(np.any(u0)), bool(np.any(t0))
+    if not (do_u or do_t):
+        return dbl, sgl, eig
+    u0 = np.asarray(u0, float).reshape(-1, 3)
+    t0 = np.asarray(t0, float).reshape(-1, 3)
+    u2 = np.asarray(u2, float).reshape(-1, 6, 3)
+    c = kb.t_coeffs(mu, lam)
+    g = kb.u_coeffs(mu, lam)
+    sp = {k: nd.spec(2, f) for k, f in (("t", nd.KT), ("h", nd.KH), ("e", nd.KE))}
+    for P, S in _near_blocks(points, cen, radius):
+        xn = np.ascontiguousarray(points[P])
+        x = torch.from_numpy(xn).to(dev)
+        Pt = torch.from_numpy(P.astype(np.int64)).to(dev)
+        src0 = dv.Sources(torch.from_numpy(tv[S]).to(dev), torch.from_numpy(nrm[S]).to(dev),
+                          torch.from_numpy(eps_arr[S]).to(dev))
+        src1 = dv.Sources(src0.tri_verts, src0.normals, torch.from_numpy(eps_near[S]).to(dev))
+        if do_u:
+            q0 = torch.from_numpy(np.ascontiguousarray(u0[S])).to(dev)
+            q2 = np.ascontiguousarray(u2[S].reshape(-1, 3))
+            el = nd.Elements(tv[S], nrm[S], eps_near[S], dev)
+            if stress:
+                blk = nd.contract(3, xn, el, sp["h"], q2, mu=mu, lam=lam)
+                dv.stress("dd", x, src0, q0, mu, lam, blk, scale=-1.0, beta=1.0)
+                dbl.index_add_(0, Pt, blk)
+                if eigen:
+                    blk = nd.contract(4, xn, el, sp["e"], q2, mu=mu, lam=lam)
+                    dv.stress("eigen", x, src0, q0, mu, lam, blk, scale=-1.0, beta=1.0)
+                    eig.index_add_(0, Pt, blk)
+            else:
+                blk = nd.contract(1, xn, el, sp["t"], q2, coeffs=c)
+                dv.t_disp(x, src0, q0, c, blk, scale=-1.0, beta=1.0)
+                dbl.index_add_(0, Pt, blk)
+        if do_t:
+            f0 = torch.from_numpy(np.ascontiguousarray(t0[S])).to(dev)
+            blk = torch.zeros((P.size,) + shape[1:], dtype=F64, device=dev)
+            if stress:
+                dv.stress("force", x, src1, f0, mu, lam, blk, scale=1.0, beta=1.0)
+                dv.stress("force", x, src0, f0, mu, lam, blk, scale=-1.0, beta=1.0)
+            else:
+                dv.u_disp(x, src1, f0, g, blk, scale=1.0, beta=1.0)
+                dv.u_disp(x, src0, f0, g, blk, scale=-1.0, beta=1.0)
+            sgl.index_add_(0, Pt, blk)
+    return dbl, sgl, eig
+
+
 def _eigen_near(ctx, points, patch, density, sigma, mu, lam, out):
     """+sigma * eigenstress over the near pairs (evaluate.py:298-326)."""
     src, tv, nrm, eps_arr = ctx.sources(patch)
@@ -200,15 +287,16 @@ def evaluate_stress(model: RegionModel, region, solution: dict, points, eps,
                     subtract_anelastic: bool = True, warn_near: bool = True,
                     parts: bool = False, near=None, near_list: bool = True):
     """Stress (N,3,3) at ``points`` of ``region`` (evaluate.py:598-695);
-    ``parts`` also returns the eigenstress removed."""
-    if near is not None:
-        raise NotImplementedError("near-face recovery (near=) is not in mbem_amd")
+    ``parts`` also returns the eigenstress removed.  ``near`` as in
+    ``evaluate_displacement``: the near pairs of every boundary patch read
+    from the recovered displacement, each double layer with its own
+    eigenstress."""
     region = _region(model, region)
     if region.faults:
         from .selfcheck import ensure_fault_convention
         ensure_fault_convention("stress")
     points = np.ascontiguousarray(np.asarray(points, dtype=float).reshape(-1, 3))
-    if warn_near:
+    if warn_near and near is None:
         _warn_near_boundary(points, region)
     ctx = _Ctx(points, eps)
     N = points.shape[0]
@@ -237,6 +325,13 @@ def evaluate_stress(model: RegionModel, region, solution: dict, points, eps,
             if np.any(t_p):
                 dv.stress("force", ctx.x, ctx.sources(p)[0], ctx.dens(t_p), mu, lam, sig,
                           scale=sigma, beta=1.0)
+            if near is not None and near.recovers(p):
+                dbl, sgl, e_near = _near_field(points, p, u_p, near.recover(p, u_p), t_p, mu, lam,
+                                               kb.resolve_patch_eps(eps, p), near, stress=True,
+                                               eigen=subtract_anelastic or parts)
+                sig += sigma * (sgl - dbl)
+                if subtract_anelastic or parts:
+                    (eig if parts else sig).add_(sigma * e_near)
         for f in region.faults:
             sigma = float(model.orientation(region, f))
             slip = _density(model, f, "slip", solution)
