@@ -121,19 +121,28 @@ class CoarseSpace:
     operator (its int8 far field and FP64 near blocks) instead of A's."""
 
     def __init__(self, assembled, chunk_dofs, basis: str = "rigid", A: torch.Tensor = None,
-                 drop: float = 1e-13, op=None):
-        if basis not in BASES:
-            raise ValueError(f"coarse basis {basis!r} (expected one of {sorted(BASES)})")
+                 drop: float = 1e-13, op=None, extra: torch.Tensor = None):
+        if basis not in BASES and basis != "none":
+            raise ValueError(f"coarse basis {basis!r} (expected one of {sorted(BASES)} or 'none')")
         _lib.require_gpu()
         A = assembled.A_dev if A is None else A
         n = A.shape[0]
-        self.n, self.basis, self.q, self.drop = n, basis, BASES[basis], drop
+        self.n, self.basis, self.drop = n, basis, drop
+        self.q = BASES.get(basis, 1)
         pts, kind, comp = dof_geometry(assembled)
-        perm, gptr, Wp = coarse_groups(chunk_dofs, pts, kind, comp, basis)
+        perm, gptr, Wp = coarse_groups(chunk_dofs, pts, kind, comp,
+                                       basis if basis != "none" else "const")
         if perm.size != n or np.any(np.bincount(perm, minlength=n) != 1):
             raise ValueError("the block-Jacobi chunks do not cover every unknown exactly once")
-        self.G = gptr.size - 1
-        K = self.G * self.q
+        self.G = gptr.size - 1 if basis != "none" else 0
+        # extra: m global columns (n x m, row-major FP64, the unknowns' own
+        # order) beside the groups' shape functions
+        self.extra = None if extra is None else extra.to(F64).contiguous()
+        if self.extra is not None and (self.extra.dim() != 2 or self.extra.shape[0] != n):
+            raise ValueError(f"extra coarse columns must be (n, m) = ({n}, m)")
+        self.m = 0 if self.extra is None else self.extra.shape[1]
+        self.K_local = self.G * self.q
+        K = self.K_local + self.m
         self.K = K
         self.perm = _lib.upload(perm, I64)
         self.Wp = _lib.upload(Wp, F64)
@@ -142,7 +151,17 @@ class CoarseSpace:
         self._gptr = _lib.upload(gptr, I64)
         dev = A.device
         self._Ct = torch.empty((K, n), dtype=F64, device=dev)
-        if op is not None and getattr(op, "i8", False):
+        if self.m:
+            # the image of the global columns: one library GEMM A E, then its
+            # transpose into the last m rows of Ct
+            AE = torch.empty((n, self.m), dtype=F64, device=dev)
+            call("mbem_dgemm_rowmajor", n, self.m, n, 1.0, ptr(A), A.stride(0), ptr(self.extra),
+                 self.m, 0.0, ptr(AE), self.m, stream())
+            self._Ct[self.K_local:].copy_(AE.t())
+            del AE
+        if self.K_local == 0:
+            self.source = "FP64 matrix (global columns only)"
+        elif op is not None and getattr(op, "i8", False):
             # the image of the mixed operator F + N itself: the int8 far
             # field (n^2 bytes) and the FP64 near blocks
             inv = np.empty(n, np.int64)
@@ -195,7 +214,7 @@ class CoarseSpace:
         self.u = torch.empty(max(K, 1), dtype=F64, device=dev)
         self.work = torch.empty(int(_lib.load().mbem_multidot_work(max(K, 1), n)), dtype=F64,
                                 device=dev)
-        self.bytes = (K * n + K * K) * 8 + self.Wp.numel() * 8 + 2 * n * 8
+        self.bytes = (K * n + K * K) * 8 + self.Wp.numel() * 8 + 2 * n * 8 + 8 * self.m * n
         return self
 
     def check(self):
@@ -221,8 +240,12 @@ class CoarseSpace:
             return z
         call("mbem_gemv", ptr(self.T), self.Kp, self.K, self.Kp, ptr(self.s), ptr(self.u), 1.0, 0.0,
              stream())
-        call("mbem_coarse_expand", ptr(self.u), ptr(self.perm), ptr(self.Wp), ptr(self.grp), self.n,
-             self.q, 1.0, ptr(z), stream())
+        if self.K_local:
+            call("mbem_coarse_expand", ptr(self.u), ptr(self.perm), ptr(self.Wp), ptr(self.grp),
+                 self.n, self.q, 1.0, ptr(z), stream())
+        if self.m:
+            call("mbem_gemv", ptr(self.extra), self.m, self.n, self.m, ptr(self.u[self.K_local:]),
+                 ptr(z), 1.0, 1.0, stream())
         return z
 
 
@@ -246,6 +269,6 @@ class TwoLevel:
     def summary(self) -> dict:
         s = self.M.summary()
         c = self.coarse.finish()
-        s["coarse"] = {"basis": c.basis, "groups": c.G, "columns": c.K, "rank": c.Kp,
-                       "bytes": c.bytes}
+        s["coarse"] = {"basis": c.basis, "groups": c.G, "columns": c.K, "global_columns": c.m,
+                       "rank": c.Kp, "bytes": c.bytes}
         return s

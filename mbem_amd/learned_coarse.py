@@ -1,0 +1,218 @@
+"""Learned coarse space: the point network of ``mbem_amd.learned`` predicts
+global deflation vectors for block-Jacobi FGMRES.
+
+The network (same architecture and kernels, csrc/learned.hip) outputs, per
+boundary triangle, K = 16 channels at each of its three unknowns: an
+n x 16 basis Phi.  Scaled by the block-Jacobi inverse diagonal D (so Phi is
+dimensionless whatever the slot kind and moduli), W = D Phi joins the
+two-level preconditioner's coarse space as 16 global columns
+(``la.coarse.CoarseSpace(extra=W)``): the part of every residual in
+range(A W) is solved exactly, z = M (r - Q s) + W T s.
+
+Training targets are the slow modes block-Jacobi leaves: after a GCRO-DR
+solve of a training geometry the recycled space Y spans the harmonic Ritz
+vectors of A M of smallest magnitude (la/gcrodr.py, after the reference's
+solver.py:248-395); V = D^-1 M Y (the same modes in the unknowns, scaled as
+Phi) is the target subspace.  The loss is the part of V outside span(Phi):
+
+    L = |V - Phi X|_F^2 / |V|_F^2,   X = (Phi^T Phi + lam I)^-1 Phi^T V,
+
+with the ridge term lam |X|^2 added (X minimises the sum, so its gradient
+is exactly dL/dPhi = -2 (V - Phi X) X^T / |V|_F^2 with lam held fixed); its
+products run in the network's low-rank kernels (``mbem_lr_coef`` /
+``mbem_lr_expand`` / ``mbem_lr_outer``), the 16 x 16 solve on the host.
+
+Training: ``python -m mbem_amd.learned_coarse --train 500 --out w.npz`` on
+the geometries of ``cases.random_box_fault(default_rng(4))``.  Forward, backward and
+Adam are the network's own HIP kernels.  The network has no counterpart in
+the reference; block-Jacobi (preconditioner.py:314-543) is its fallback.
+"""
+from __future__ import annotations
+
+import math
+import os
+import time
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import call, ptr, stream
+from .learned import RANK, Cloud, PointNet
+
+F32, F64 = torch.float32, torch.float64
+WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "learned_coarse.npz")
+
+
+def _nslices(P):
+    return min(256, max(1, P // 64))
+
+
+def basis_rows(net: PointNet, cl: Cloud, D: torch.Tensor = None, out_fp64: bool = True):
+    """The network's Phi (or D Phi) as RANK rows of length 3P (after a forward)."""
+    P, K = cl.P, RANK
+    eye = torch.eye(K, dtype=F64, device=cl.dev)
+    rows = torch.empty((K, 3 * P), dtype=F64 if out_fp64 else F32, device=cl.dev)
+    call("mbem_lr_expand", ptr(cl.y), P, K, 0, ptr(eye), K, ptr(rows), 3 * P, int(out_fp64), 1.0,
+         0, ptr(D), stream())
+    return rows
+
+
+def subspace_loss_grad(net: PointNet, cl: Cloud, Vt: torch.Tensor, dy: torch.Tensor,
+                       ridge: float = 1e-6, lam: float = None) -> float:
+    """The ridge objective (|V - Phi X|^2 + lam |X|^2) / |V|^2 of the network
+    output cl.y against the target rows Vt (B x 3P, FP32), X its minimiser,
+    lam = ridge tr(Phi^T Phi) / K unless given (held fixed in the
+    gradient), and dL/dy = -2 (V - Phi X) X^T / |V|^2 into dy (P x 6K, the
+    Phi columns; the rest zero)."""
+    P, K = cl.P, RANK
+    B, n = Vt.shape
+    ns = _nslices(P)
+    part = torch.empty(ns * max(B, K) * K, dtype=F64, device=cl.dev)
+    Phi_t = basis_rows(net, cl, out_fp64=False)                     # K x n
+    G = torch.empty((K, K), dtype=F64, device=cl.dev)
+    call("mbem_lr_coef", ptr(cl.y), P, K, 0, ptr(Phi_t), n, K, 0, ptr(part), ns, ptr(G), stream())
+    BV = torch.empty((B, K), dtype=F64, device=cl.dev)
+    call("mbem_lr_coef", ptr(cl.y), P, K, 0, ptr(Vt), n, B, 0, ptr(part), ns, ptr(BV), stream())
+    Gh = G.cpu().numpy()
+    Gh = 0.5 * (Gh + Gh.T)
+    if lam is None:
+        lam = ridge * max(np.trace(Gh) / K, 1e-30)
+    X = np.linalg.solve(Gh + lam * np.eye(K), BV.cpu().numpy().T)   # K x B
+    Xt = torch.as_tensor(np.ascontiguousarray(X.T), dtype=F64, device=cl.dev)   # B x K
+    pred = torch.empty((B, n), dtype=F32, device=cl.dev)
+    call("mbem_lr_expand", ptr(cl.y), P, K, 0, ptr(Xt), B, ptr(pred), n, 0, 1.0, 0, None, stream())
+    R = Vt - pred
+    v2 = float((Vt.double() ** 2).sum())
+    loss = (float((R.double() ** 2).sum()) + lam * float((X * X).sum())) / v2
+    dpred = (R * (-2.0 / v2)).contiguous()
+    dy.zero_()
+    call("mbem_lr_outer", ptr(dpred), n, ptr(Xt), B, P, K, 0, ptr(dy), stream())
+    return loss
+
+
+@dataclass
+class ModeSample:
+    cloud: Cloud
+    Vt: torch.Tensor          # B x 3P FP32 target rows (unit norm)
+
+
+def slow_modes(asm, M, k: int = RANK, rtol: float = 1e-8, restart: int = 200, maxiter: int = 600):
+    """(V rows k x n FP32, report): the D^-1-scaled, normalised M Y of the
+    recycled space of one GCRO-DR solve of the assembled system."""
+    from .la.solver import RecycleSpace, fgmres
+    rs = RecycleSpace(k)
+    _, rep = fgmres(asm, asm.b_dev, M=M, rtol=rtol, restart=restart, maxiter=maxiter, recycle=rs)
+    Y = rs.vectors
+    if Y is None:
+        raise RuntimeError("the solve left no recycled space")
+    D = M.inverse_diagonal().abs().clamp_min(1e-300)
+    rows = []
+    for j in range(Y.shape[1]):
+        z = torch.empty_like(asm.b_dev)
+        M.apply_into(Y[:, j].contiguous(), z)
+        v = z / D
+        rows.append(v / torch.linalg.vector_norm(v))
+    return torch.stack(rows).float().contiguous(), rep
+
+
+def train(net: PointNet, samples, epochs: int = 20, lr: float = 1e-3, seed: int = 0, log=print,
+          lr_final: float = 0.1):
+    """Adam over the samples, one step per geometry; mean loss per epoch."""
+    rng = np.random.default_rng(seed)
+    hist = []
+    for ep in range(epochs):
+        t0 = time.perf_counter()
+        tot = 0.0
+        for i in rng.permutation(len(samples)):
+            s = samples[i]
+            net.forward(s.cloud)
+            dy = torch.zeros_like(s.cloud.y)
+            tot += subspace_loss_grad(net, s.cloud, s.Vt, dy)
+            net.zero_grad()
+            net.backward(s.cloud, dy)
+            frac = ep / max(epochs - 1, 1)
+            net.adam(lr=lr * (lr_final + (1 - lr_final) * 0.5 * (1 + math.cos(math.pi * frac))))
+        hist.append(tot / max(len(samples), 1))
+        log(f"[learned-coarse] epoch {ep}: mean loss {hist[-1]:.4f} ({time.perf_counter() - t0:.1f} s)")
+    return hist
+
+
+def evaluate_loss(net: PointNet, samples) -> float:
+    tot = 0.0
+    for s in samples:
+        net.forward(s.cloud)
+        dy = torch.zeros_like(s.cloud.y)
+        tot += subspace_loss_grad(net, s.cloud, s.Vt, dy)
+    return tot / max(len(samples), 1)
+
+
+def coarse_columns(M, layout, net: PointNet = None, bf16: bool = True):
+    """(W (n x 16) = D Phi, inference seconds) for a mesh: one forward of the
+    network (BF16 matrix cores) over the mesh's point cloud."""
+    net = net or PointNet.load(WEIGHTS)
+    t0 = time.perf_counter()
+    cl = Cloud(layout)
+    net.forward(cl, bf16=bf16)
+    D = M.inverse_diagonal().abs().clamp_min(1e-300).contiguous()
+    W = basis_rows(net, cl, D=D).t().contiguous()
+    torch.cuda.current_stream().synchronize()
+    return W, time.perf_counter() - t0
+
+
+def preconditioner(asm, M, basis: str = "none", net: PointNet = None, bf16: bool = True):
+    """Two-level block-Jacobi with the learned global columns (and, with
+    ``basis`` "const" / "rigid" / "linear", the classical per-chunk shape
+    functions beside them)."""
+    from .la.coarse import CoarseSpace, TwoLevel
+    W, infer_s = coarse_columns(M, asm.layout, net=net, bf16=bf16)
+    P = TwoLevel(M, CoarseSpace(asm, M.chunk_dofs, basis=basis, extra=W).finish())
+    P.inference_s = infer_s
+    return P
+
+
+def main(argv=None):
+    """Build the targets of --train geometries, train, save the weights."""
+    import argparse
+    import json
+
+    from .learned_train import build_geometry, free
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--train", type=int, default=500)
+    ap.add_argument("--cells", type=int, nargs=2, default=(26, 50))
+    ap.add_argument("--seed", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--bj-chunk", type=int, default=3000)
+    ap.add_argument("--rtol", type=float, default=1e-10)
+    ap.add_argument("--out", default=None)
+    args = ap.parse_args(argv)
+    _lib.require_gpu()
+    rng = np.random.default_rng(args.seed)
+    clouds, samples = {}, []
+    t0 = time.perf_counter()
+    for g in range(args.train):
+        asm, M, cl, info = build_geometry(rng, args.cells, args.bj_chunk, clouds)
+        Vt, rep = slow_modes(asm, M, rtol=args.rtol)
+        samples.append(ModeSample(cl, Vt))
+        print(f"[coarse-data] geometry {g}: n_cells {info['n_cells']}, {rep.iterations} "
+              f"iterations, {Vt.shape[0]} modes ({time.perf_counter() - t0:.0f} s)", flush=True)
+        del M
+        free(asm)
+    data_s = time.perf_counter() - t0
+    net = PointNet()
+    t0 = time.perf_counter()
+    hist = train(net, samples, epochs=args.epochs, lr=args.lr, log=lambda m: print(m, flush=True))
+    train_s = time.perf_counter() - t0
+    if args.out:
+        net.save(args.out, loss_history=np.asarray(hist), seed=args.seed, n_train=args.train,
+                 kind="coarse")
+    summary = {"train_geometries": args.train, "epochs": args.epochs, "loss_history": hist,
+               "data_s": data_s, "train_s": train_s}
+    print(json.dumps(summary), flush=True)
+    return summary
+
+
+if __name__ == "__main__":
+    main()

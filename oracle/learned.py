@@ -98,6 +98,24 @@ def lowrank_loss(y, R, T):
     return loss, dy
 
 
+def subspace_loss(y, V, lam):
+    """(loss, dy) of the learned coarse space's ridge objective
+        L = (|V^T - Phi X|_F^2 + lam |X|_F^2) / |V|_F^2,
+        X = (Phi^T Phi + lam I)^-1 Phi^T V^T,
+    the part of the target rows V (B x 3P) outside span(Phi).  X minimises
+    the bracket, so dL/dPhi = -2 (V^T - Phi X) X^T / |V|^2 exactly (lam held
+    fixed); dy carries it in the Phi columns, the Psi columns zero."""
+    Phi, _ = bases(y)
+    K = Phi.shape[1]
+    X = np.linalg.solve(Phi.T @ Phi + lam * np.eye(K), Phi.T @ V.T)
+    R = V.T - Phi @ X
+    v2 = float((V * V).sum())
+    loss = (float((R * R).sum()) + lam * float((X * X).sum())) / v2
+    dy = np.zeros_like(y)
+    dy[:, :3 * K] = (-2.0 * R @ X.T / v2).reshape(y.shape[0], 3 * K)
+    return loss, dy
+
+
 def backward(theta, cache, y, t, s, dy=None):
     """Gradient with respect to theta (flat, same order) of ``loss(y, t, s)``,
     or, given ``dy``, of any loss whose gradient in y is dy."""

@@ -96,3 +96,29 @@ def test_adam_step_matches_algorithm_1():
     vh = (1 - b2) * g * g / (1 - b2)
     ref = w - lr * mh / (np.sqrt(vh) + eps / np.sqrt(1 - b2))
     assert np.allclose(w1, ref, rtol=1e-12, atol=1e-14)
+
+
+def test_subspace_loss_gradient_matches_finite_differences():
+    """The learned coarse space's ridge objective (|V - Phi X|^2 + lam |X|^2)
+    / |V|^2 with X its minimiser: the analytic gradient -2 (V - Phi X) X^T
+    equals central differences, and the loss of a Phi whose span holds V is
+    only the ridge term."""
+    rng = np.random.default_rng(8)
+    P, B = 13, 4
+    y = rng.normal(size=(P, ol.N_OUT))
+    V = rng.normal(size=(B, 3 * P))
+    lam = 0.3
+    L, dy = ol.subspace_loss(y, V, lam)
+    assert 0.0 < L < 1.0
+    h = 1e-6
+    for idx in rng.choice(y.size, size=60, replace=False):
+        yp, ym = y.copy(), y.copy()
+        yp.flat[idx] += h
+        ym.flat[idx] -= h
+        fd = (ol.subspace_loss(yp, V, lam)[0] - ol.subspace_loss(ym, V, lam)[0]) / (2 * h)
+        assert abs(fd - dy.flat[idx]) <= 1e-6 * max(1.0, abs(fd)) + 1e-9
+    assert not np.any(dy[:, 3 * ol.RANK:])
+    Phi, _ = ol.bases(y)
+    Vin = (Phi @ rng.normal(size=(ol.RANK, B))).T
+    L0, _ = ol.subspace_loss(y, Vin, 1e-12)
+    assert L0 < 1e-9
