@@ -134,7 +134,8 @@ class CoarseSpace:
                                        basis if basis != "none" else "const")
         if perm.size != n or np.any(np.bincount(perm, minlength=n) != 1):
             raise ValueError("the block-Jacobi chunks do not cover every unknown exactly once")
-        self.G = gptr.size - 1 if basis != "none" else 0
+        ngroups = gptr.size - 1
+        self.G = ngroups if basis != "none" else 0
         # extra: m global columns (n x m, row-major FP64, the unknowns' own
         # order) beside the groups' shape functions
         self.extra = None if extra is None else extra.to(F64).contiguous()
@@ -146,7 +147,7 @@ class CoarseSpace:
         self.K = K
         self.perm = _lib.upload(perm, I64)
         self.Wp = _lib.upload(Wp, F64)
-        grp = np.repeat(np.arange(self.G, dtype=np.int64), np.diff(gptr))
+        grp = np.repeat(np.arange(ngroups, dtype=np.int64), np.diff(gptr))
         self.grp = _lib.upload(grp, I64)
         self._gptr = _lib.upload(gptr, I64)
         dev = A.device
