@@ -172,23 +172,139 @@ def preconditioner(asm, M, basis: str = "none", net: PointNet = None, bf16: bool
     return P
 
 
+def _group_columns(V, D, perm, gptr, q):
+    """(n x G q) local coarse columns: per (chunk, kind) group the top q
+    principal directions of the rows of V (S x n, D^-1 scaled) restricted to
+    the group, scaled back by D and zero outside the group."""
+    n = V.shape[1]
+    cols = []
+    for g in range(gptr.size - 1):
+        idx = perm[gptr[g]:gptr[g + 1]]
+        _, _, vh = np.linalg.svd(V[:, idx], full_matrices=False)
+        for c in range(min(q, vh.shape[0])):
+            w = np.zeros(n)
+            w[idx] = vh[c] * D[idx]
+            cols.append(w)
+    return np.stack(cols, axis=1)
+
+
+def coarse_bounds(asm, M, seed: int = 0, rtol: float = 1e-8) -> dict:
+    """Iterations and seconds of FGMRES(200) to ``rtol`` on one assembled
+    system with block-Jacobi ``M`` and coarse spaces that bound what a
+    network predicting coarse columns from the geometry could reach, since
+    each is computed from this system's own operator:
+
+    * ``exact{k}``: the k slow modes of a GCRO-DR solve with a random
+      right-hand side (D^-1 M Y, the learned columns' training target) as
+      global columns, alone and beside the rigid functions;
+    * ``pod_slow{q}``: per (chunk, kind) group the top q principal
+      directions of 48 such modes;
+    * ``pod_err{q}``: per group the top q principal directions of 48 errors
+      x* - x left by one FGMRES(12) cycle from random x* (the components
+      block-Jacobi and Krylov leave), beside the rigid functions;
+
+    against block-Jacobi alone and the fixed rigid / linear bases."""
+    from .la.coarse import CoarseSpace, TwoLevel, coarse_groups, dof_geometry
+    from .la.solver import fgmres
+    n = asm.layout.n_unknowns
+    dev = asm.A_dev.device
+    out = {"n": n}
+
+    def run(label, P):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, rep = fgmres(asm, asm.b_dev, M=P, rtol=rtol, restart=200, maxiter=600)
+        torch.cuda.synchronize()
+        out[label] = [int(rep.iterations), round(time.perf_counter() - t0, 4)]
+
+    def two_level(basis, extra=None):
+        return TwoLevel(M, CoarseSpace(asm, M.chunk_dofs, basis=basis, extra=extra).finish())
+
+    run("block_jacobi", M)
+    run("rigid", two_level("rigid"))
+    run("linear", two_level("linear"))
+    gen = torch.Generator(device=dev).manual_seed(seed)
+    b_keep = asm.b_dev.clone()
+    asm.b_dev.copy_(torch.randn(n, dtype=F64, device=dev, generator=gen))
+    Vt, _ = slow_modes(asm, M, k=48, rtol=1e-10)
+    asm.b_dev.copy_(b_keep)
+    Dd = M.inverse_diagonal().abs().clamp_min(1e-300)
+    Dh = Dd.cpu().numpy()
+    for k in (16, 48):
+        W = (Vt[:k].double() * Dd[None, :]).t().contiguous()
+        run(f"exact{k}", two_level("none", W))
+        run(f"rigid+exact{k}", two_level("rigid", W))
+    pts, kind, comp = dof_geometry(asm)
+    perm, gptr, _ = coarse_groups(M.chunk_dofs, pts, kind, comp, "const")
+    V = Vt.double().cpu().numpy()
+    W = torch.as_tensor(_group_columns(V, Dh, perm, gptr, 9), device=dev)
+    run("rigid+pod_slow9", two_level("rigid", W))
+    errs = []
+    for _ in range(48):
+        xs = torch.randn(n, dtype=F64, device=dev, generator=gen) * Dd
+        b = torch.empty_like(xs)
+        asm.apply_into(xs, b)
+        x, _ = fgmres(asm, b, M=M, rtol=1e-30, restart=12, maxiter=12)
+        e = (xs - x) / Dd
+        errs.append((e / torch.linalg.vector_norm(e)).cpu().numpy())
+    E = np.stack(errs)
+    for q in (6, 12):
+        W = torch.as_tensor(_group_columns(E, Dh, perm, gptr, q), device=dev)
+        run(f"rigid+pod_err{q}", two_level("rigid", W))
+    return out
+
+
+def bounds_main(n_geometries: int, seed: int = 4, n_train: int = 500):
+    """coarse_bounds on the first held-out geometries of the training family
+    (the draws of ``random_box_fault_params(default_rng(seed))`` after the
+    ``n_train`` training draws); one JSON line per geometry and the means."""
+    import json
+
+    from .backends.dense import AssembledDense
+    from .cases import random_box_fault, random_box_fault_params
+    from .learned_train import free
+    from .model import generate_system
+    _lib.require_gpu()
+    rng = np.random.default_rng(seed)
+    for _ in range(n_train):
+        random_box_fault_params(rng)
+    rows = []
+    for g in range(n_geometries):
+        model, eps, info = random_box_fault(None, params=random_box_fault_params(rng))
+        system = generate_system(model)
+        asm = AssembledDense(system, eps, "direct", jump="calibrated")
+        M = asm.preconditioner(0, 3000, grouping="region")
+        row = coarse_bounds(asm, M, seed=g)
+        rows.append(row)
+        print(json.dumps({"geometry": g, "n_cells": info["n_cells"], **row}), flush=True)
+        del M
+        free(asm)
+    mean = {k: float(np.mean([r[k][0] for r in rows])) for k in rows[0] if k != "n"}
+    print(json.dumps({"mean_iterations": mean}), flush=True)
+    return rows, mean
+
+
 def main(argv=None):
-    """Build the targets of --train geometries, train, save the weights."""
+    """Build the targets of --train geometries, train, save the weights
+    (or, with --bounds G, measure coarse_bounds on G held-out geometries)."""
     import argparse
     import json
 
     from .learned_train import build_geometry, free
     ap = argparse.ArgumentParser()
+    ap.add_argument("--bounds", type=int, default=0)
     ap.add_argument("--train", type=int, default=500)
     ap.add_argument("--cells", type=int, nargs=2, default=(26, 50))
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--epochs", type=int, default=30)
+    ap.add_argument("--epochs", type=int, default=20)
     ap.add_argument("--lr", type=float, default=1e-3)
     ap.add_argument("--bj-chunk", type=int, default=3000)
     ap.add_argument("--rtol", type=float, default=1e-10)
     ap.add_argument("--out", default=None)
     args = ap.parse_args(argv)
     _lib.require_gpu()
+    if args.bounds:
+        return bounds_main(args.bounds, seed=args.seed, n_train=args.train)
     rng = np.random.default_rng(args.seed)
     clouds, samples = {}, []
     t0 = time.perf_counter()
