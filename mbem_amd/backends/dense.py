@@ -49,6 +49,20 @@ from ..model.equations import BlockSystem, add_jump_rhs, rhs_terms_at
 F64 = torch.float64
 
 
+# Streams of the assembly's concurrent block launches: a few per device for
+# the life of the process (default priority).
+_ASM_STREAMS: dict = {}
+ASM_STREAMS_MAX = 4
+
+
+def _assembly_streams(k: int) -> list:
+    dev = torch.cuda.current_device()
+    pool = _ASM_STREAMS.setdefault(dev, [])
+    while len(pool) < k:
+        pool.append(torch.cuda.Stream(device=dev))
+    return pool[:k]
+
+
 class DenseBackend:
     """dense.py:49-86.  ``jump="calibrated"`` annihilates constant fields
     exactly; ``deflate`` borders out the rigid translations of an all-Neumann
@@ -471,6 +485,33 @@ class AssembledDense:
                     if (id(r), id(q)) in need:
                         calib_dev[(id(r), id(q))] = torch.zeros((q.n_nodes, 3, 3), dtype=F64,
                                                                 device=dev)
+        b = torch.zeros(n, dtype=F64, device=dev)
+        # Concurrent launches: each block kernel is only a round or two of
+        # waves deep, so on one stream every block ends in a tail of idle
+        # SIMDs.  Blocks of different field patches (collocation rows) write
+        # disjoint rows of A and b and disjoint row-sum buffers, so each field
+        # patch gets a stream of its own and the others' blocks fill its
+        # tails; blocks of one field patch (shared row sums, accumulated
+        # blocks, the free term after its row sums) keep their order.
+        main = torch.cuda.current_stream()
+        rows = []
+        for item in list(self.system.terms) + [rt for rt, _ in rhs_in]:
+            if id(item.field_patch) not in [id(p) for p in rows]:
+                rows.append(item.field_patch)
+        for r in model.regions:
+            for q in r.patches:
+                if (id(r), id(q)) in calib_dev and id(q) not in [id(p) for p in rows]:
+                    rows.append(q)
+        pool = _assembly_streams(max(1, min(ASM_STREAMS_MAX, len(rows))))
+        row_stream = {id(p): pool[i % len(pool)] for i, p in enumerate(rows)}
+        ready = torch.cuda.Event()
+        ready.record(main)
+        for s_ in pool:
+            s_.wait_event(ready)
+
+        def on(patch):
+            return torch.cuda.stream(row_stream[id(patch)])
+
         covered = set()
         stored_T = set()
         for t in self.system.terms:
@@ -482,18 +523,19 @@ class AssembledDense:
             src = self._src[id(t.source_patch)]
             out = A[r0:, c0:]
             c = self._coeffs(t.kernel, t.region)
-            if t.kernel == KERNEL_T:
-                ck = (id(t.region), id(t.field_patch))
-                rs = calib_dev.get(ck) if (ck in calib_dev) else None
-                dv.t_block(x, src, c, out, lda, scale=t.scale, beta=beta, rowsum=rs,
-                           rs_scale=-float(model.orientation(t.region, t.source_patch)))
-                stored_T.add((id(t.region), id(t.field_patch), id(t.source_patch)))
-                n_ts += x.shape[0] * src.ns
-            elif t.kernel == KERNEL_U:
-                dv.u_block(x, src, c, out, lda, scale=t.scale, beta=beta)
-                n_u += x.shape[0] * src.ns
-            else:
-                raise NotImplementedError(f"kernel {t.kernel!r} is not in mbem_amd")
+            with on(t.field_patch):
+                if t.kernel == KERNEL_T:
+                    ck = (id(t.region), id(t.field_patch))
+                    rs = calib_dev.get(ck) if (ck in calib_dev) else None
+                    dv.t_block(x, src, c, out, lda, scale=t.scale, beta=beta, rowsum=rs,
+                               rs_scale=-float(model.orientation(t.region, t.source_patch)))
+                    stored_T.add((id(t.region), id(t.field_patch), id(t.source_patch)))
+                    n_ts += x.shape[0] * src.ns
+                elif t.kernel == KERNEL_U:
+                    dv.u_block(x, src, c, out, lda, scale=t.scale, beta=beta)
+                    n_u += x.shape[0] * src.ns
+                else:
+                    raise NotImplementedError(f"kernel {t.kernel!r} is not in mbem_amd")
         # zero the blocks no term writes
         slots = self.layout.slots
         for rs_ in slots:
@@ -512,8 +554,10 @@ class AssembledDense:
                             continue
                         x = self._pts[id(q)]
                         src = self._src[id(p)]
-                        dv.t_block(x, src, self._coeffs(KERNEL_T, r), None, 0, rowsum=calib_dev[ck],
-                                   rs_scale=-float(model.orientation(r, p)), store=False)
+                        with on(q):
+                            dv.t_block(x, src, self._coeffs(KERNEL_T, r), None, 0,
+                                       rowsum=calib_dev[ck],
+                                       rs_scale=-float(model.orientation(r, p)), store=False)
                         n_tr += x.shape[0] * src.ns
             self.calib_dev = calib_dev
         # free terms on the u-block diagonals
@@ -522,24 +566,30 @@ class AssembledDense:
                 continue
             blk = A[t.row.offset:, t.col.offset:]
             nq = t.field_patch.n_nodes
-            if self.jump == "calibrated":
-                D = calib_dev[(id(t.region), id(t.field_patch))]
-            else:
-                D = torch.as_tensor(np.broadcast_to(t.diag * np.eye(3), (nq, 3, 3)).copy(),
-                                    dtype=F64, device=dev)
-            dv.add_block_diag(blk, lda, D)
+            with on(t.field_patch):
+                if self.jump == "calibrated":
+                    D = calib_dev[(id(t.region), id(t.field_patch))]
+                else:
+                    D = torch.as_tensor(np.broadcast_to(t.diag * np.eye(3), (nq, 3, 3)).copy(),
+                                        dtype=F64, device=dev)
+                dv.add_block_diag(blk, lda, D)
         # right-hand side (dense.py:514-524)
-        b = torch.zeros(n, dtype=F64, device=dev)
         for rt, dens in rhs_in:
             x = self._pts[id(rt.field_patch)]
             src = self._src[id(rt.source_patch)]
             out = b[rt.row.offset:rt.row.stop].view(-1, 3)
             c = self._coeffs(rt.kernel, rt.region)
-            if rt.kernel == KERNEL_T:
-                dv.t_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
-            else:
-                dv.u_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
+            with on(rt.field_patch):
+                if rt.kernel == KERNEL_T:
+                    dv.t_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
+                else:
+                    dv.u_disp(x, src, dens, c, out, scale=rt.scale, beta=1.0)
             n_rhs += x.shape[0] * src.ns
+        # join: the rest runs on the caller's stream behind every block
+        for s_ in pool:
+            done = torch.cuda.Event()
+            done.record(s_)
+            main.wait_event(done)
         for rv, vec in rv_in:
             b[rv.row.offset:rv.row.stop] += vec
         # prescribed-displacement rows: the free term times u_bar moves to the
