@@ -2,8 +2,8 @@
 # Kernel identifiers, as in the reference (kernels/__init__.py:4-12).
 KERNEL_T = "H"     # T-kernel: displacement jump (slip / u_p) -> displacement
 KERNEL_U = "G"     # U-kernel: traction -> displacement
-KERNEL_HT = "Ht"   # jump -> elastic traction (FAULT_TRACTION rows; not in mbem_amd)
-KERNEL_GT = "Gt"   # traction -> traction (FAULT_TRACTION rows; not in mbem_amd)
+KERNEL_HT = "Ht"   # jump -> elastic traction (FAULT_TRACTION rows: basis.assemble_traction_matrix)
+KERNEL_GT = "Gt"   # traction -> traction (FAULT_TRACTION rows: basis.assemble_traction_matrix)
 KERNELS = (KERNEL_T, KERNEL_U, KERNEL_HT, KERNEL_GT)
 DISPLACEMENT_KERNELS = (KERNEL_T, KERNEL_U)
 
