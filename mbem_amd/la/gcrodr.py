@@ -145,13 +145,15 @@ class GcroDR:
     def __init__(self, n, restart, dim, dev):
         from .solver import _Krylov
         self.n, self.m, self.dim, self.dev = n, restart, dim, dev
-        self.ws = _Krylov(n, restart, dev)
         kmax = dim + 2
+        # the C-projection coefficients of an Arnoldi step sit in the tail of
+        # the CGS2 buffer: both reach the host in one transfer per step
+        self.ws = _Krylov(n, restart, dev, tail=kmax)
         self.sets = [_Bases(kmax, n, dev), _Bases(kmax, n, dev)]
         self.T1 = torch.zeros((kmax, n), dtype=F64, device=dev)
         self.T2 = torch.zeros((kmax, n), dtype=F64, device=dev)
         self.ops = _Ops(n, max(restart + 1, kmax), dev)
-        self.hb = torch.empty(kmax, dtype=F64, device=dev)
+        self.hb = self.ws.tail_view(kmax)
 
     # ---- (2.7): A U = C with C orthonormal --------------------------------
     def _orthonormalize(self, AU, k, src_u, src_y, dst):
@@ -240,8 +242,10 @@ class GcroDR:
                 if k:                                        # (I - C C^T) A M
                     ops.dots_dev(cur.C, k, w, self.hb)
                     ops.axpy(cur.C, k, self.hb[:k], w, -1.0)
-                    Bk[:, j] = self.hb[:k].cpu().numpy()
-                hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True)
+                    hsum, wnorm, bk = ws.cgs2(V, j + 1, w, normalize=True, tail=k)
+                    Bk[:, j] = bk
+                else:
+                    hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True)
                 H[:j + 1, j] = hsum
                 H[j + 1, j] = wnorm
                 if not wnorm > 1e-300:

@@ -410,21 +410,22 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
         side.wait_event(gathered)
         with torch.cuda.stream(side):
             built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pat)
+            if coarse:
+                # the coarse space behind the operator build on the side
+                # stream, queued before the inversions' long launch sequence,
+                # so its image pass and factorisation run beside the
+                # inversions: the image of the mixed operator itself (int8
+                # codes and near blocks; for BF16 / FP32 storage the FP64 A,
+                # whose near blocks the build has put back by then)
+                from .coarse import CoarseSpace
+                built["cs"] = CoarseSpace(assembled, chunk_dofs, basis=coarse,
+                                          op=built["op"]).finish()
 
     M = BlockGaussSeidel(assembled, max_dense=precond_max_dense, bj_chunk=precond_bj_chunk,
                          grouping=precond_grouping, inverse_dtype="f32", defer_check=True,
                          after_gather=start_operator)
     op = built["op"]
-    cs = None
-    if coarse:
-        # queued after the inversions, so its host work (the groups and
-        # their shape values) runs while the device inverts; on the side
-        # stream behind the operator build: the image of the mixed operator
-        # itself (int8 codes and near blocks; for BF16 / FP32 storage the
-        # FP64 A, whose near blocks the build has put back by then)
-        from .coarse import CoarseSpace
-        with torch.cuda.stream(side):
-            cs = CoarseSpace(assembled, M.chunk_dofs, basis=coarse, op=op)
+    cs = built.get("cs")
     t1 = time.perf_counter()
     main.wait_stream(side)
     M.coupling = op                      # the lower couplings read the mixed operator

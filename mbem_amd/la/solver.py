@@ -157,11 +157,14 @@ def gemv(A: torch.Tensor, x: torch.Tensor, out=None, alpha=1.0, beta=0.0):
 class _Krylov:
     """Device workspace and the three vector passes of one FGMRES."""
 
-    def __init__(self, n, m, dev):
+    def __init__(self, n, m, dev, tail: int = 0):
         self.n = n
         self.V = torch.empty((m + 1, n), dtype=F64, device=dev)
         self.Z = torch.empty((m, n), dtype=F64, device=dev)
-        self.h = torch.empty(2 * (m + 1) + 1, dtype=F64, device=dev)
+        # CGS2 coefficients and norm in h[:2 (m + 1) + 1]; ``tail`` more
+        # entries behind them (``tail_view``) travel to the host with them
+        self.h_len = 2 * (m + 1) + 1
+        self.h = torch.empty(self.h_len + tail, dtype=F64, device=dev)
         self.work = torch.empty(int(_lib.load().mbem_multidot_work(m + 1, n)), dtype=F64,
                                 device=dev)
 
@@ -172,11 +175,19 @@ class _Krylov:
              ptr(self.work), stream())
         return out
 
-    def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor, normalize: bool = False):
+    def tail_view(self, e: int) -> torch.Tensor:
+        """The first e entries of h's tail (a device buffer whose values
+        ``cgs2(..., tail=e)`` returns with its own)."""
+        return self.h[self.h_len:self.h_len + e]
+
+    def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor, normalize: bool = False,
+             tail: int = 0):
         """Project w against V[:k] twice (classical Gram-Schmidt, reorthogonalized)
         and return (h, ||w||) on the host after ONE synchronization; with
         ``normalize`` w is also divided by its norm on the device (when the
-        norm exceeds 1e-300) before that synchronization."""
+        norm exceeds 1e-300) before that synchronization.  With ``tail`` = e
+        the first e values of ``tail_view`` come back in the same transfer,
+        as a third element."""
         h1 = self.dots(V, k, w, 0)
         self.axpy_rows(V, k, h1, w, -1.0)
         h2 = self.dots(V, k, w, k)
@@ -184,8 +195,12 @@ class _Krylov:
         self.dots(w.view(1, -1), 1, w, 2 * k)
         if normalize:
             call("mbem_normalize", self.n, ptr(self.h[2 * k:]), ptr(w), stream())
-        hv = self.h[:2 * k + 1].cpu().numpy()
-        return hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0)))
+        if not tail:
+            hv = self.h[:2 * k + 1].cpu().numpy()
+            return hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0)))
+        hv = self.h[:self.h_len + tail].cpu().numpy()
+        return (hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0))),
+                hv[self.h_len:self.h_len + tail].copy())
 
     def axpy_rows(self, rows: torch.Tensor, k: int, coef: torch.Tensor, w: torch.Tensor,
                   alpha: float):
