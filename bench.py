@@ -696,10 +696,18 @@ def main():
         for k in range(steps):
             asm_k = x = None
             _release(full=k == 0)
-            ts = time.perf_counter()
-            asm_k, x, rep = step_fn()
-            torch.cuda.synchronize()
-            times.append(time.perf_counter() - ts)
+            # Python's cyclic collector is off inside a timed step, as the
+            # standard library's timeit does, and runs between steps
+            # (_release): a full collection over the solve's objects in the
+            # middle of a step stalls the host for tens of milliseconds
+            gc.disable()
+            try:
+                ts = time.perf_counter()
+                asm_k, x, rep = step_fn()
+                torch.cuda.synchronize()
+                times.append(time.perf_counter() - ts)
+            finally:
+                gc.enable()
             errs.append(float(torch.linalg.vector_norm(x - x_ref)
                               / torch.linalg.vector_norm(x_ref)))
             iters.append(rep.iterations)

@@ -188,11 +188,82 @@ def device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
+class _StagingRing:
+    """One pinned host buffer that every upload stages through, as a ring.
+
+    Each upload writes its bytes at the ring's head (a plain memcpy), queues
+    the copy to the device on the current stream and records an event after
+    it; a region is written again only once its event has completed, so the
+    host waits only if it laps a copy the device has not reached.  A fresh
+    pinned tensor per upload instead goes through torch's pinned-block cache,
+    whose blocks cannot be reused while their copies are still queued (the
+    usual case here: uploads are queued behind the assembly), so it pins new
+    memory per upload, at 0.3 ms each on MI355X and up to several ms now and
+    then.  Uploads larger than half the ring take that path."""
+
+    CAPACITY = 64 << 20
+    ALIGN = 256
+
+    def __init__(self):
+        self.buf = torch.empty(self.CAPACITY, dtype=torch.uint8, pin_memory=True)
+        self.host = self.buf.numpy()
+        self.head = 0
+        self.pending = []               # (start, end, event), oldest first
+
+    def _claim(self, nbytes: int) -> int:
+        start = self.head
+        if start + nbytes > self.CAPACITY:
+            start = 0
+        end = start + nbytes
+        keep = []
+        for s, e, ev in self.pending:
+            if s < end and start < e:
+                ev.synchronize()        # the device has not copied this region yet
+            elif not ev.query():
+                keep.append((s, e, ev))
+        self.pending = keep
+        self.head = end + (-end % self.ALIGN)
+        return start
+
+    def upload(self, a: np.ndarray, dtype) -> torch.Tensor:
+        npdt = _np_dtype(dtype)
+        nbytes = a.size * npdt.itemsize
+        dev = device()
+        if nbytes == 0:
+            return torch.empty(a.shape, dtype=dtype, device=dev)
+        start = self._claim(nbytes)
+        stage = self.host[start:start + nbytes].view(npdt).reshape(a.shape)
+        np.copyto(stage, a, casting="unsafe")
+        out = torch.empty(a.shape, dtype=dtype, device=dev)
+        out.copy_(self.buf[start:start + nbytes].view(dtype).view(a.shape), non_blocking=True)
+        ev = torch.cuda.Event()
+        ev.record()
+        self.pending.append((start, start + nbytes, ev))
+        return out
+
+
+_NP_DTYPES = {}
+_ring = None
+
+
+def _np_dtype(dtype) -> np.dtype:
+    if dtype not in _NP_DTYPES:
+        _NP_DTYPES[dtype] = torch.empty(0, dtype=dtype).numpy().dtype
+    return _NP_DTYPES[dtype]
+
+
 def upload(a, dtype):
     """A host array on the device through a pinned staging copy, without
-    waiting for the work already queued on the stream."""
-    t = torch.as_tensor(np.ascontiguousarray(a), dtype=dtype).pin_memory()
-    return t.to(device(), non_blocking=True)
+    waiting for the work already queued on the stream (the staging ring's
+    region is reused only after the device has copied it)."""
+    global _ring
+    a = np.ascontiguousarray(a)         # (a 0-d array becomes shape (1,), as torch saw it)
+    if a.size * _np_dtype(dtype).itemsize > _StagingRing.CAPACITY // 2:
+        t = torch.as_tensor(a, dtype=dtype).pin_memory()
+        return t.to(device(), non_blocking=True)
+    if _ring is None:
+        _ring = _StagingRing()
+    return _ring.upload(a, dtype)
 
 
 _raw_stream = None
