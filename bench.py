@@ -10,9 +10,12 @@ One step = a whole GPU solve from the mesh arrays: dense assembly
 over cluster chunks, explicit chunk inverses) and the Krylov solve to the
 relative-error target against an FP64 reference solution computed once,
 untimed, at rtol 1e-13.  Two methods are stepped: plain FP64 (block-Jacobi
-FGMRES) and mixed (BF16 far field + FP64 near field inner FGMRES with FP32
-block-Jacobi and a recycled Krylov space, under an FP64 outer FGMRES);
-``value`` is the mixed step, the plain step and the ratio are in ``config``.
+FGMRES) and mixed (int8 far field + FP64 near field inner FGMRES with FP32
+block-Jacobi, a rigid coarse space and a recycled Krylov space, under an
+FP64 outer FGMRES whose operator is, by default, the int8 codes plus their
+32-bit rounding remainders: every far entry within 1e-12 of its segment's
+largest, from 5 bytes instead of 8); ``value`` is the mixed step, the plain
+step and the ratio are in ``config``.
 The learned preconditioner's A/B on the same system is reported beside it.
 
 ``--workload laplace`` (batched Laplace-state solves) and ``--workload
@@ -67,6 +70,13 @@ def i8_bytes(n: int) -> int:
     """Algorithmic bytes of one int8 far-field matvec (n x n): the codes, one
     FP32 scale per row and 64 columns, x in FP32, y in FP64."""
     return n * n + 4 * n * (-(-n // 64)) + 4 * n + 8 * n
+
+
+def i8e_bytes(n: int) -> int:
+    """Algorithmic bytes of one split far-field matvec (n x n): the int8
+    codes, the 32-bit remainders, one FP32 scale per row and 64 columns, x
+    and y in FP64."""
+    return n * n + 4 * n * n + 4 * n * (-(-n // 64)) + 8 * n + 8 * n
 
 
 def _release(full: bool = False):
@@ -582,6 +592,9 @@ def main():
     ap.add_argument("--grouping-mixed", choices=("patch", "region"), default="region")
     ap.add_argument("--coarse", choices=("none", "const", "rigid", "linear"), default="rigid",
                     help="coarse space of the mixed method's two-level block-Jacobi")
+    ap.add_argument("--outer", choices=("fp64", "split"), default="split",
+                    help="operator of the mixed method's outer loop with int8 storage: the FP64 "
+                         "matrix, or the int8 codes plus their 32-bit rounding remainders")
     ap.add_argument("--other-storage-steps", type=int, default=3,
                     help="steps of the mixed method with the other far-field storage (BF16 "
                          "beside int8), reported beside the headline")
@@ -660,14 +673,23 @@ def main():
     y = torch.empty(n, dtype=torch.float64, device=A.device)
     gemv_ms = _events_time(lambda: gemv(A, x_ref, y), 10)
     gemv_bytes = n * n * 8 + 2 * n * 8
-    far = None
+    far = outer_far = None
     if args.method in ("mixed", "both"):
         # the mixed method's far-field matvec, the mixed step's dominant kernel
         from mbem_amd._lib import call, ptr, stream
         from mbem_amd.la.mixed import MixedOperator
-        op = MixedOperator(asm, storage=args.storage)
+        split = args.storage == "int8" and args.outer == "split"
+        op = MixedOperator(asm, storage=args.storage, split=split)
         call("mbem_vec_to_f32", ptr(x_ref), n, ptr(op.xf), stream())
         far_ms = _events_time(lambda: op._far(0, n, n, 1.0, 0.0, y), 10)
+        if split:
+            # the outer loop's split far-field matvec, timed the same way
+            outer_ms = _events_time(
+                lambda: call("mbem_gemv_i8e", ptr(op.F), op.ldf, ptr(op.E), op.lde, ptr(op.S),
+                             op.nseg, n, n, ptr(x_ref), ptr(y), 1.0, 0.0, stream()), 10)
+            outer_far = ("k_gemv_i8e", outer_ms, i8e_bytes(n))
+            print(f"[bench] k_gemv_i8e (split outer) {outer_ms:.3f} ms = "
+                  f"{i8e_bytes(n) / outer_ms / 1e6:.0f} GB/s", flush=True)
         far_bytes = {"int8": i8_bytes(n), "bf16": 2 * n * n + 12 * n,
                      "fp32": 4 * n * n + 12 * n}[args.storage]
         kname = {"int8": "k_gemv_i8", "bf16": "k_gemv_lo", "fp32": "k_gemv_lo"}[args.storage]
@@ -730,10 +752,12 @@ def main():
     def step_mixed(storage=None):
         from mbem_amd.la.mixed import solve_mixed
         a = AssembledDense(system, eps, "direct", jump="calibrated")
-        x, rep = solve_mixed(a, rtol=args.rtol, storage=storage or args.storage,
+        st = storage or args.storage
+        x, rep = solve_mixed(a, rtol=args.rtol, storage=st,
                              inner_rtol=args.inner_rtol, precond_bj_chunk=args.bj_chunk_mixed,
                              precond_grouping=args.grouping_mixed, recycle=args.recycle,
-                             coarse=(None if args.coarse == "none" else args.coarse))
+                             coarse=(None if args.coarse == "none" else args.coarse),
+                             outer=(args.outer if st == "int8" else "fp64"))
         a.timings.update(mixed_operator_s=rep.operator_build_s,
                          precond_s=rep.precond_build_s, fgmres_s=rep.solve_s)
         return a, x, rep
@@ -813,6 +837,12 @@ def main():
                              f"mixed far-field matvec, n x n {args.storage}", ref=far[3:])
     else:
         roof_main = roof_fp64
+    mixed_outer = args.outer if args.storage == "int8" else "fp64"
+    outer_desc = ("operator: the FP64 matrix" if mixed_outer == "fp64" else
+                  "operator: int8 codes + 32-bit rounding remainders, FP64 sums")
+    roof_outer = (roofline(outer_far[0], outer_far[1], outer_far[2],
+                           "mixed outer matvec, n x n int8 codes + 32-bit remainders, FP64 sums")
+                  if main_label == "mixed" and outer_far is not None else None)
     line = {
         "metric": METRIC,
         "value": ms_per_step / 1e3,
@@ -831,8 +861,10 @@ def main():
                         f"(n={n}), dipping fault {info['fault_triangles']} triangles, "
                         f"eps=0.5h, mu=30 GPa nu=0.25",
             "method": (f"mixed: {args.storage.upper()} far field + FP64 near field on an FP64 "
-                       f"outer FGMRES, FP32 block-Jacobi" if main_label == "mixed" else
+                       f"outer FGMRES ({outer_desc}), FP32 block-Jacobi"
+                       if main_label == "mixed" else
                        "plain FP64: GPU assembly + block-Jacobi FGMRES"),
+            "outer_operator": (mixed_outer if main_label == "mixed" else "fp64"),
             "storage": args.storage if main_label == "mixed" else "fp64",
             "rtol": args.rtol, "rel_err_vs_fp64_reference": max(errs),
             "target_rel_err": args.target_err, "iterations": iters,
@@ -854,6 +886,7 @@ def main():
         },
         "roofline": roof_main,
         "roofline_fp64_matvec": roof_fp64,
+        "roofline_outer_matvec": roof_outer,
         "roofline_assembly": asm_rate,
         "cpu_baseline": cpu,
     }

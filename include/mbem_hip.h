@@ -53,6 +53,9 @@ extern "C" {
 #define MBEM_I8_ROWS_PER_WAVE 8
 #define MBEM_I8_LOADS_PER_ROW 2
 
+/* rows of the split int8 far field one wavefront multiplies (mbem_gemv_i8e) */
+#define MBEM_I8E_ROWS_PER_WAVE 4
+
 /* doubles per source record */
 int mbem_src_fields(void);
 
@@ -289,6 +292,15 @@ int64_t mbem_i8_segments(int64_t n);
 int mbem_cast_i8(const double *A, int64_t lda, int64_t m, int64_t n, uint8_t *out, int64_t ldo,
                  float *scale, int64_t nseg, hipStream_t stream);
 
+/* mbem_cast_i8 plus the code's rounding remainder: resid[r * ldr + c] =
+ * e = rint((A - s q) / s * 2^32) clamped to +-(2^31 - 1), so that
+ * s (q + e 2^-32) is A to within s 2^-33 (1e-12 of the segment's largest
+ * entry).  resid 16-byte aligned, ldr a multiple of 4.  Same role as
+ * mbem_cast_i8 (la/solver.py:178 operator, stored in 5 bytes an entry);
+ * its product is mbem_gemv_i8e. */
+int mbem_cast_i8e(const double *A, int64_t lda, int64_t m, int64_t n, uint8_t *out, int64_t ldo,
+                  float *scale, int64_t nseg, int32_t *resid, int64_t ldr, hipStream_t stream);
+
 /* Hold the near blocks out of A while its int8 copy is built: restore == 0
  * copies them to vals (9 doubles per block, as mbem_near_extract) and zeroes
  * them in A; restore != 0 writes vals back, leaving A as it was. */
@@ -301,6 +313,15 @@ int mbem_near_hold(double *A, int64_t lda, int64_t nbr, const int64_t *rowptr, c
 int mbem_gemv_i8(const uint8_t *F, int64_t ldf, const float *scale, int64_t nseg, int64_t m,
                  int64_t n, const float *x, double *y, double alpha, double beta,
                  hipStream_t stream);
+
+/* y = alpha F x + beta y for the split far field of mbem_cast_i8e,
+ * F[r, c] = scale * (out[r, c] - 128 + resid[r, c] 2^-32), FP64 x (16-byte
+ * aligned) and FP64 sums: the FGMRES operator of la/solver.py:178 (A v) with
+ * the near field added by mbem_bsr_gemv, read from 5 bytes an entry instead
+ * of the FP64 matrix's 8. */
+int mbem_gemv_i8e(const uint8_t *F, int64_t ldf, const int32_t *resid, int64_t ldr,
+                  const float *scale, int64_t nseg, int64_t m, int64_t n, const double *x,
+                  double *y, double alpha, double beta, hipStream_t stream);
 
 /* y += alpha N x for a 3x3 block-CSR N in FP64 (the near field). */
 int mbem_bsr_gemv(int64_t nbr, const int64_t *rowptr, const int64_t *col, const double *vals,

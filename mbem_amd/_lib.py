@@ -75,6 +75,8 @@ _SIGS = {
     "mbem_cast_i8": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p],
     "mbem_near_hold": [_p, _i64, _i64, _p, _p, _i64, _p, _int, _p],
     "mbem_gemv_i8": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
+    "mbem_cast_i8e": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p],
+    "mbem_gemv_i8e": [_p, _i64, _p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
     "mbem_bsr_gemv": [_i64, _p, _p, _p, _p, _p, _d, _p],
     "mbem_gather_block_f32": [_p, _i64, _p, _i64, _p, _p],
     "mbem_invert_batched_f32": [_p, _i64, _i64, _p, _p, _p],

@@ -257,13 +257,21 @@ __global__ __launch_bounds__(256) void k_gemv_lo(const void* __restrict__ F, lon
 // errs by a fixed fraction of each entry, largest on the largest entries).
 constexpr long kSeg = 64;
 
+// With E, the code's rounding remainder is kept too: d = A - s q is exact in
+// FP64 (s q has at most 31 significant bits; for q != 0, A and s q are within
+// a factor 2 of each other), and e = rint(d / s * 2^32), |e| <= 2^31 - 1,
+// so s (q + e 2^-32) is A to within s 2^-33 (and FP64's rounding of d / s):
+// about 1e-12 of the segment's largest entry, from 5 bytes instead of 8.
+//
 // one wave per (row, 1024 columns), in 4 passes of 256: lane l encodes 4
 // consecutive columns, so a 64-column segment is one 16-lane row of the wave
 // and its largest magnitude is a reduction inside that row; loads (32 bytes
-// a lane) and 4-byte stores are contiguous across the wave
+// a lane), 4-byte code stores and 16-byte remainder stores are contiguous
+// across the wave
 __global__ __launch_bounds__(256) void k_cast_i8(const double* __restrict__ A, long lda, long m,
                                                  long n, unsigned char* __restrict__ out, long ldo,
-                                                 float* __restrict__ scale, long nseg)
+                                                 float* __restrict__ scale, long nseg,
+                                                 int* __restrict__ E, long lde)
 {
     const int lane = threadIdx.x & 63;
     const long nblk = (n + 1023) / 1024;
@@ -294,18 +302,29 @@ __global__ __launch_bounds__(256) void k_cast_i8(const double* __restrict__ A, l
         amax = fmax(amax, __shfl_xor(amax, 8, 64));
         const float s = (float)(amax / 127.0);
         unsigned int q[4];
+        int e[4];
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             double t = (s > 0.0f) ? rint(v[i] / (double)s) : 0.0;
             t = fmin(fmax(t, -127.0), 127.0);
             q[i] = (unsigned int)((int)t + 128);
+            double f = (s > 0.0f) ? rint((v[i] - (double)s * t) / (double)s * 4294967296.0) : 0.0;
+            f = fmin(fmax(f, -2147483647.0), 2147483647.0);
+            e[i] = (int)f;
         }
         if (c + 3 < n) {
             *reinterpret_cast<unsigned int*>(o + c) = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
+            if (E) {
+                u32x4 ev = {(unsigned)e[0], (unsigned)e[1], (unsigned)e[2], (unsigned)e[3]};
+                *reinterpret_cast<u32x4*>(E + r * lde + c) = ev;
+            }
         } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (c + i < n) o[c + i] = (unsigned char)q[i];
+                if (c + i < n) {
+                    o[c + i] = (unsigned char)q[i];
+                    if (E) E[r * lde + c + i] = e[i];
+                }
         }
         if ((lane & 15) == 0 && c < n) scale[r * nseg + c / kSeg] = s;
     }
@@ -462,6 +481,138 @@ __global__ __launch_bounds__(256) void k_gemv_i8(const unsigned char* __restrict
         if (lane == 0 && r0 + r < m) {
             double* p = y + r0 + r;
             *p = (beta == 0.0) ? alpha * (double)t : alpha * (double)t + beta * (*p);
+        }
+    }
+}
+
+// sum over a group of four columns of (u - 128 + e 2^-32) x in FP64, given
+// sx = the group's sum of x: the codes' offset is taken out once per group
+__device__ __forceinline__ double group_i8e(const unsigned w, const u32x4 e, const dbl2 xa,
+                                            const dbl2 xb, const double sx)
+{
+    const double ux = (double)(w & 0xffu) * xa.x + (double)((w >> 8) & 0xffu) * xa.y +
+                      (double)((w >> 16) & 0xffu) * xb.x + (double)(w >> 24) * xb.y;
+    const double ex = (double)(int)e.x * xa.x + (double)(int)e.y * xa.y + (double)(int)e.z * xb.x +
+                      (double)(int)e.w * xb.y;
+    return (ux - 128.0 * sx) + ex * 0x1p-32;
+}
+
+// y = alpha (F x) + beta y in FP64 for the split far field
+// F[r, c] = s (u - 128 + e 2^-32) (codes u, remainders e, the per-segment
+// scales s of k_cast_i8), FP64 x: the outer operator of the mixed method.
+// One wave per R rows.  In each 1024-column step lane l takes four groups of
+// four columns, c = c0 + 256 g + 4 l (g < 4): per group and row one 4-byte
+// load of codes and one 16-byte load of remainders, contiguous across the
+// wave (256 and 1,024 bytes).  A group lies in segment c0 / 64 + 4 g + l / 16;
+// a row's 16 scales of the step are one load (lane l holds segment
+// c0 / 64 + l % 16) and each group takes its own by a lane permute.  x goes
+// through LDS 2048 columns at a time, double-buffered as in k_gemv_i8 (the
+// same barrier discipline: every wave runs the same tile loop, rows past m
+// are clamped and not stored).  Requires 256 threads per block.
+template <int R>
+__global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restrict__ F, long ldf,
+                                                  const int* __restrict__ E, long lde,
+                                                  const float* __restrict__ S, long nseg, long m,
+                                                  long n, const double* __restrict__ x,
+                                                  double* __restrict__ y, double alpha, double beta)
+{
+    constexpr unsigned kTile = 2048;            // columns of x per LDS buffer
+    __shared__ dbl2 xs[2][kTile / 2];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long r0 = ((long)blockIdx.x * (blockDim.x >> 6) + wv) * R;
+    const unsigned char* fb[R];
+    const int* eb[R];
+    const float* sb[R];
+    double acc[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        acc[r] = 0.0;
+        const long rr = (r0 + r < m) ? r0 + r : m - 1;
+        fb[r] = F + rr * ldf;
+        eb[r] = E + rr * lde;
+        sb[r] = S + rr * nseg;
+    }
+    // columns within a row fit 32 bits (the entry point checks n < 2^31)
+    const unsigned ntile = (unsigned)(n / kTile);
+    if (ntile > 0) {
+        const dbl2* xp = reinterpret_cast<const dbl2*>(x);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) xs[0][threadIdx.x + 256 * i] = xp[threadIdx.x + 256 * i];
+        __syncthreads();
+    }
+    for (unsigned k = 0; k < ntile; ++k) {
+        const unsigned base = k * kTile;
+        const bool more = k + 1 < ntile;
+        dbl2 xn[4];
+        if (more) {
+            const dbl2* xp = reinterpret_cast<const dbl2*>(x + base + kTile);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xn[i] = xp[threadIdx.x + 256 * i];
+        }
+        const dbl2* xt = xs[k & 1];
+        // the two steps of a tile one after the other: unrolled, both steps'
+        // loads are hoisted together and R = 4 rows take all 256 VGPRs (one
+        // wave per SIMD) instead of 150 (three)
+#pragma unroll 1
+        for (int st = 0; st < 2; ++st) {
+            const unsigned c0 = base + 1024 * st;
+            unsigned w[4][R];
+            u32x4 ev[4][R];
+            float sv[R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                sv[r] = sb[r][c0 / 64 + (lane & 15)];
+#pragma unroll
+                for (int g = 0; g < 4; ++g) {
+                    const unsigned c = c0 + 256 * g + 4 * lane;
+                    w[g][r] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(fb[r] + c));
+                    ev[g][r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(eb[r] + c));
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                const dbl2* xl = xt + (1024 * st + 256 * g + 4 * lane) / 2;
+                const dbl2 xa = xl[0], xb = xl[1];
+                const double sx = (xa.x + xa.y) + (xb.x + xb.y);
+                const int src = 4 * g + (lane >> 4);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    acc[r] += (double)__shfl(sv[r], src, 64) * group_i8e(w[g][r], ev[g][r], xa, xb, sx);
+            }
+        }
+        if (more) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) xs[(k + 1) & 1][threadIdx.x + 256 * i] = xn[i];
+            __syncthreads();
+        }
+    }
+    // columns past the last tile: groups of four, x from global memory
+    const unsigned ng = (unsigned)(n / 4);
+    for (unsigned q = ntile * (kTile / 4) + lane; q < ng; q += 64) {
+        const unsigned c = 4 * q;
+        const dbl2* xp = reinterpret_cast<const dbl2*>(x + c);
+        const dbl2 xa = xp[0], xb = xp[1];
+        const double sx = (xa.x + xa.y) + (xb.x + xb.y);
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[r] += (double)sb[r][c / 64] *
+                      group_i8e(*reinterpret_cast<const unsigned*>(fb[r] + c),
+                                *reinterpret_cast<const u32x4*>(eb[r] + c), xa, xb, sx);
+    }
+    for (unsigned c = 4 * ng + lane; c < (unsigned)n; c += 64) {
+        const double xc = x[c];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+            acc[r] += (double)sb[r][c / 64] *
+                      ((((double)fb[r][c] - 128.0) + (double)eb[r][c] * 0x1p-32) * xc);
+    }
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const double t = wave_sum_d(acc[r]);
+        if (lane == 0 && r0 + r < m) {
+            double* p = y + r0 + r;
+            *p = (beta == 0.0) ? alpha * t : alpha * t + beta * (*p);
         }
     }
 }
@@ -746,7 +897,28 @@ int mbem_cast_i8(const double* A, int64_t lda, int64_t m, int64_t n, uint8_t* ou
     const long waves = m * ((n + 1023) / 1024);
     const long blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffL) return MBEM_EINVAL;
-    hipLaunchKernelGGL(k_cast_i8, dim3((unsigned)blocks), dim3(256), 0, stream, A, lda, m, n, out, ldo, scale, nseg);
+    hipLaunchKernelGGL(k_cast_i8, dim3((unsigned)blocks), dim3(256), 0, stream, A, lda, m, n, out, ldo, scale, nseg,
+                       (int*)nullptr, 0L);
+    return launch_status();
+}
+
+int mbem_cast_i8e(const double* A, int64_t lda, int64_t m, int64_t n, uint8_t* out, int64_t ldo,
+                  float* scale, int64_t nseg, int32_t* resid, int64_t ldr, hipStream_t stream)
+{
+    if (m < 0 || n < 0 ||
+        (m > 0 && n > 0 && (!A || !out || !scale || !resid || lda < n || ldo < n || ldr < n)))
+        return MBEM_EINVAL;
+    if (nseg != mbem_i8_segments(n)) return MBEM_EINVAL;
+    if (m == 0 || n == 0) return MBEM_OK;
+    // 16-byte stores of the packed codes and of the remainders
+    if ((reinterpret_cast<uintptr_t>(out) & 15) || (ldo % 16) ||
+        (reinterpret_cast<uintptr_t>(resid) & 15) || (ldr % 4))
+        return MBEM_EINVAL;
+    const long waves = m * ((n + 1023) / 1024);
+    const long blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffL) return MBEM_EINVAL;
+    hipLaunchKernelGGL(k_cast_i8, dim3((unsigned)blocks), dim3(256), 0, stream, A, lda, m, n, out, ldo, scale, nseg,
+                       resid, (long)ldr);
     return launch_status();
 }
 
@@ -764,6 +936,30 @@ int mbem_gemv_i8(const uint8_t* F, int64_t ldf, const float* scale, int64_t nseg
     long waves = (m + R - 1) / R;
     dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     hipLaunchKernelGGL((k_gemv_i8<R, MBEM_I8_LOADS_PER_ROW>), grid, block, 0, stream, F, ldf, scale, nseg, m, n, x, y, alpha, beta);
+    return launch_status();
+}
+
+int mbem_gemv_i8e(const uint8_t* F, int64_t ldf, const int32_t* resid, int64_t ldr,
+                  const float* scale, int64_t nseg, int64_t m, int64_t n, const double* x,
+                  double* y, double alpha, double beta, hipStream_t stream)
+{
+    if (m < 0 || n < 0 || (m > 0 && (!F || !resid || !y || !scale || ldf < n || ldr < n)) ||
+        (n > 0 && !x))
+        return MBEM_EINVAL;
+    if (nseg < mbem_i8_segments(n)) return MBEM_EINVAL;
+    if (m == 0) return MBEM_OK;
+    // 4-byte code loads and 16-byte remainder and x loads at multiples of 4 columns
+    if ((reinterpret_cast<uintptr_t>(F) & 15) || (ldf % 16) ||
+        (reinterpret_cast<uintptr_t>(resid) & 15) || (ldr % 4) ||
+        (reinterpret_cast<uintptr_t>(x) & 15))
+        return MBEM_EINVAL;
+    if (n > INT32_MAX) return MBEM_EINVAL;     // 32-bit column offsets
+    constexpr int R = MBEM_I8E_ROWS_PER_WAVE;
+    const long waves = (m + R - 1) / R;
+    const long blocks = (waves + 3) / 4;
+    if (blocks > 0x7fffffffL) return MBEM_EINVAL;
+    hipLaunchKernelGGL((k_gemv_i8e<R>), dim3((unsigned)blocks), dim3(256), 0, stream, F, ldf, resid, ldr,
+                       scale, nseg, m, n, x, y, alpha, beta);
     return launch_status();
 }
 

@@ -9,12 +9,16 @@ Operator split A = F + N on the device:
      multiplied with FP32 accumulation (int8: offset-binary bytes with one
      FP32 scale per row and 64-column segment).
 
-Solve: the outer loop is FP64 FGMRES on the exact FP64 operator (Arnoldi,
-Givens and the true residual all FP64), so it is iterative refinement; its
+Solve: the outer loop is FP64 FGMRES on the FP64 operator (Arnoldi, Givens
+and the true residual all FP64), so it is iterative refinement -- either the
+assembled FP64 matrix or, with int8 storage, ``SplitOuter``: the int8 codes
+plus their 32-bit rounding remainders, every far entry within 1e-12 of its
+segment's largest magnitude, from 5 bytes instead of 8; its
 right preconditioner is an inner FGMRES on the mixed operator F + N,
 preconditioned by block Gauss-Seidel / block-Jacobi with FP32 chunk
 inverses and mixed-operator couplings, run to a loose tolerance.  The error
-against the FP64 solution is set by the outer rtol alone.
+against the FP64 solution is set by the outer rtol (the split operator's
+entry errors, about 1e-12 of each segment's largest, stay far below it).
 """
 from __future__ import annotations
 
@@ -94,9 +98,12 @@ def near_structure(assembled, k_near: float = 2.0):
 class MixedOperator:
     """y = F x + N x on the device (see the module docstring)."""
 
-    def __init__(self, assembled, storage: str = "bf16", k_near: float = 2.0, pattern=None):
+    def __init__(self, assembled, storage: str = "bf16", k_near: float = 2.0, pattern=None,
+                 split: bool = False):
         if storage not in ("bf16", "fp32", "int8"):
             raise ValueError(f"storage {storage!r} (expected 'bf16', 'fp32' or 'int8')")
+        if split and storage != "int8":
+            raise ValueError("the split far field (codes + remainders) is an int8 storage")
         _lib.require_gpu()
         A = assembled.A_dev
         n = A.shape[0]
@@ -122,11 +129,19 @@ class MixedOperator:
             self.nseg = int(_lib.load().mbem_i8_segments(n))
             self.F = torch.empty((n, self.ldf), dtype=torch.uint8, device=dev)
             self.S = torch.empty((n, max(self.nseg, 1)), dtype=torch.float32, device=dev)
+            # with split, each code's rounding remainder too (32 bits): the
+            # outer operator of SplitOuter
+            self.lde = ((n + 3) // 4) * 4
+            self.E = torch.empty((n, self.lde), dtype=torch.int32, device=dev) if split else None
             hold = (self.nbr, ptr(self.rowptr), ptr(self.col), self.nnzb, ptr(self.vals))
             call("mbem_near_hold", ptr(A), A.stride(0), *hold, 0, stream())
             try:
-                call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
-                     ptr(self.S), self.nseg, stream())
+                if split:
+                    call("mbem_cast_i8e", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
+                         ptr(self.S), self.nseg, ptr(self.E), self.lde, stream())
+                else:
+                    call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
+                         ptr(self.S), self.nseg, stream())
             finally:
                 # the near blocks go back into the caller's A whatever the
                 # cast returned: A is never left with them zeroed
@@ -141,6 +156,7 @@ class MixedOperator:
                  ptr(self.col), self.nnzb, ptr(self.vals), ptr(self.F), self.ldf, int(self.bf16),
                  stream())
             self.far_bytes = n * n * (2 if self.bf16 else 4)
+            self.E = None
         self._lower = {}
         self.xf = torch.empty(max(n, 4), dtype=torch.float32, device=dev)   # FP32 copy of x
         # host time only: the casts are queued, not waited for (stream order
@@ -194,6 +210,34 @@ class MixedOperator:
         nbr, rowptr, col, vals = self._lower_bsr(r0, r1)
         call("mbem_bsr_gemv", nbr, ptr(rowptr), ptr(col), ptr(vals), ptr(z), ptr(out), -1.0,
              stream())
+
+
+class SplitOuter:
+    """The outer loop's operator from the split far field: A v as
+    s (u - 128 + e 2^-32) v over the far entries (int8 codes u and 32-bit
+    remainders e, FP64 v and FP64 sums) plus the FP64 near blocks.  Every far
+    entry is A's to within s 2^-33 (s = its segment's largest magnitude /
+    127, so about 1e-12 of that magnitude; the near blocks are exact), read
+    from 5 bytes instead of 8: the FP64 pass over A in each outer step of the
+    solve (la/solver.py:178) becomes a pass over 5/8 of the bytes."""
+
+    def __init__(self, op: MixedOperator):
+        if op.E is None:
+            raise ValueError("SplitOuter needs a MixedOperator built with split=True")
+        self.op = op
+        self.n = op.n
+        self.bytes = op.n * op.ldf + op.n * op.lde * 4 + op.n * op.nseg * 4 + op.near_bytes
+
+    def apply_into(self, v, out):
+        o = self.op
+        call("mbem_gemv_i8e", ptr(o.F), o.ldf, ptr(o.E), o.lde, ptr(o.S), o.nseg, o.n, o.n,
+             ptr(v), ptr(out), 1.0, 0.0, stream())
+        call("mbem_bsr_gemv", o.nbr, ptr(o.rowptr), ptr(o.col), ptr(o.vals), ptr(v), ptr(out),
+             1.0, stream())
+        return out
+
+    def matvec(self, v):
+        return self.apply_into(v, torch.empty_like(v))
 
 
 class _InnerSolve:
@@ -348,6 +392,8 @@ class MixedReport:
     near_blocks: int = 0
     far_bytes: int = 0
     coarse: dict | None = None
+    outer_operator: str = "fp64"
+
 
     @property
     def converged(self):
@@ -360,7 +406,9 @@ class MixedReport:
     def __str__(self):
         c = "" if not self.coarse else (f", coarse {self.coarse['basis']} rank "
                                         f"{self.coarse['rank']}")
-        return (f"mixed ({self.storage} far field, FP64 near + outer{c}): outer {self.outer}, "
+        o = ("FP64 outer" if self.outer_operator == "fp64" else
+             "outer on int8 codes + 32-bit remainders, FP64 sums")
+        return (f"mixed ({self.storage} far field, FP64 near, {o}{c}): outer {self.outer}, "
                 f"inner iterations {self.inner_iterations}")
 
 
@@ -382,11 +430,20 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
                 inner_rtol: float = 5e-2, inner_maxiter: int = 60, outer_restart: int = 30,
                 outer_maxiter: int = 40, precond_max_dense=0, precond_bj_chunk=1500,
                 precond_grouping: str = "region", pattern=None, recycle: int = 90,
-                coarse: str = None):
+                coarse: str = None, outer: str = "fp64"):
     """Mixed-precision FGMRES of ``assembled``; returns (x device, MixedReport).
     ``recycle`` > 0 keeps up to that many recycled vectors across the inner
     solves (``_RecycledInner``); ``coarse`` ("const", "rigid", "linear")
-    adds the coarse correction of la/coarse.py to the inner block-Jacobi."""
+    adds the coarse correction of la/coarse.py to the inner block-Jacobi.
+    ``outer`` is the outer loop's operator: "fp64" the assembled FP64 matrix,
+    "split" (int8 storage) ``SplitOuter``, the int8 codes plus their 32-bit
+    rounding remainders (every far entry within 1e-12 of its segment's
+    largest, FP64 sums, near blocks exact), 5/8 of the FP64 bytes per outer
+    step."""
+    if outer not in ("fp64", "split"):
+        raise ValueError(f"outer {outer!r} (expected 'fp64' or 'split')")
+    if outer == "split" and storage != "int8":
+        raise ValueError("outer='split' needs storage='int8'")
     from .preconditioner import BlockGaussSeidel
     # Two streams (both at the default priority).  The near pattern needs
     # the geometry only: its counting pass is queued on a side stream beside
@@ -409,7 +466,8 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
         pat = near.result() if near is not None else pattern
         side.wait_event(gathered)
         with torch.cuda.stream(side):
-            built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pat)
+            built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pat,
+                                        split=outer == "split")
             if coarse:
                 # the coarse space behind the operator build on the side
                 # stream, queued before the inversions' long launch sequence,
@@ -440,12 +498,13 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
                                kmax=recycle)
     else:
         inner = _InnerSolve(op, P, inner_rtol, inner_maxiter)
-    x, outer = fgmres(assembled, assembled.b_dev, M=inner, rtol=rtol, restart=outer_restart,
-                      maxiter=outer_maxiter)
+    A_outer = SplitOuter(op) if outer == "split" else assembled
+    x, outer_rep = fgmres(A_outer, assembled.b_dev, M=inner, rtol=rtol, restart=outer_restart,
+                          maxiter=outer_maxiter)
     torch.cuda.current_stream().synchronize()
     t3 = time.perf_counter()
-    rep = MixedReport(outer=outer, inner_iterations=inner.iterations, operator_build_s=t2 - t1,
+    rep = MixedReport(outer=outer_rep, inner_iterations=inner.iterations, operator_build_s=t2 - t1,
                       precond_build_s=t1 - t0, solve_s=t3 - t2, storage=storage,
                       near_blocks=op.nnzb, far_bytes=op.far_bytes,
-                      coarse=(P.summary()["coarse"] if coarse else None))
+                      coarse=(P.summary()["coarse"] if coarse else None), outer_operator=outer)
     return x, rep

@@ -9,6 +9,12 @@ does, so the GPU bytes, scales and products can be checked bit for bit:
   q = rint(A / s) (ties to even, the division in float64) clamped to
   [-127, 127], q = 0 where s = 0; the byte is q + 128; the represented value
   is s (byte - 128).
+
+The split form (mbem_cast_i8e, the mixed method's outer operator) keeps each
+code's rounding remainder as well: e = rint((A - s q) / s * 2^32) (the
+difference, the division and the product by 2^32 in float64), clamped to
++-(2^31 - 1), e = 0 where s = 0; the represented value is
+s (byte - 128 + e 2^-32).
 """
 from __future__ import annotations
 
@@ -44,3 +50,24 @@ def dequantize(codes, scales) -> np.ndarray:
     n = codes.shape[1]
     seg = np.arange(n) // SEG
     return (codes.astype(np.float64) - 128.0) * scales[:, seg].astype(np.float64)
+
+
+def cast_split(A) -> tuple:
+    """(codes, scales, remainders (m, n) int32) of a float64 matrix."""
+    A = np.asarray(A, dtype=np.float64)
+    codes, scales = cast(A)
+    n = A.shape[1]
+    sd = scales[:, np.arange(n) // SEG].astype(np.float64)
+    q = codes.astype(np.float64) - 128.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        e = np.where(sd > 0, np.rint((A - sd * q) / np.where(sd > 0, sd, 1.0) * 4294967296.0), 0.0)
+    e = np.clip(e, -2147483647.0, 2147483647.0).astype(np.int32)
+    return codes, scales, e
+
+
+def dequantize_split(codes, scales, resid) -> np.ndarray:
+    """float64 matrix the split storage represents."""
+    n = codes.shape[1]
+    seg = np.arange(n) // SEG
+    v = (codes.astype(np.float64) - 128.0) + resid.astype(np.float64) * 2.0 ** -32
+    return v * scales[:, seg].astype(np.float64)

@@ -100,3 +100,20 @@ def test_capi_rejects_bad_arguments_without_a_device():
         _lib.call("mbem_gemv", null, 0, -1, 0, null, null, 1.0, 0.0, null)
     with pytest.raises(_lib.MbemError, match="HIP error"):
         _lib.check(_lib.MBEM_EHIP, "probe")
+    # the split far field's entry points: missing remainders, a remainder
+    # pitch that breaks the 16-byte loads; an empty product is valid
+    assert lib.mbem_cast_i8e(null, 4, 4, 4, null, 16, null, 1, null, 4, null) == _lib.MBEM_EINVAL
+    assert lib.mbem_gemv_i8e(null, 16, null, 6, null, 1, 4, 4, null, null, 1.0, 0.0,
+                             null) == _lib.MBEM_EINVAL
+    assert lib.mbem_gemv_i8e(null, 16, null, 4, null, 0, 0, 0, null, null, 1.0, 0.0,
+                             null) == _lib.MBEM_OK
+
+
+def test_split_outer_option_is_checked():
+    """The split outer operator exists for the int8 storage only, and the
+    option is checked before anything touches the device."""
+    from mbem_amd.la.mixed import solve_mixed
+    with pytest.raises(ValueError, match="storage='int8'"):
+        solve_mixed(None, storage="bf16", outer="split")
+    with pytest.raises(ValueError, match="outer"):
+        solve_mixed(None, storage="int8", outer="fp32")

@@ -48,3 +48,58 @@ def test_segment_extremes_ties_and_zeros():
     assert scales[2, 0] == 0 and np.all(codes[2] == 128)
     assert scales[0, 2] == 0 and np.all(codes[0, 128:] == 128)
     assert np.all(oi.dequantize(codes, scales)[2] == 0)
+
+
+def test_split_remainder_is_the_nearest_integer_exactly():
+    """The remainder e of the split form against exact rational arithmetic:
+    t = (A - s q) / s * 2^32 computed with fractions, |e - t| <= 1/2 (plus
+    float64's rounding of the division, below 2^-20), so the represented
+    value s (q + e 2^-32) is A to within s 2^-33 (and float64's rounding)
+    on every entry of a matrix spanning 30 decades."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    A = rng.standard_normal((6, 400)) * np.exp(rng.uniform(-35, 5, 400))[None, :]
+    A[3] *= 1e-9
+    codes, scales, resid = oi.cast_split(A)
+    assert resid.dtype == np.int32 and resid.shape == A.shape
+    for r in range(6):
+        for c in range(0, 400, 7):
+            s = Fraction(float(scales[r, c // 64]))
+            q = int(codes[r, c]) - 128
+            t = (Fraction(float(A[r, c])) - s * q) / s * 2 ** 32
+            assert abs(int(resid[r, c]) - t) <= Fraction(1, 2) + Fraction(1, 2 ** 20), (r, c)
+    D = oi.dequantize_split(codes, scales, resid)
+    step = scales[:, np.arange(400) // 64].astype(np.float64)
+    assert np.all(np.abs(D - A) <= step * 2.0 ** -33 * (1 + 1e-6) + np.abs(A) * 2.0 ** -52)
+    # the codes are the int8 form's, unchanged
+    c8, s8 = oi.cast(A)
+    assert np.array_equal(codes, c8) and np.array_equal(scales, s8)
+
+
+def test_split_remainder_known_answers():
+    """Entries built as s (q + k 2^-32) exactly (s = 1/8: the segment holds
+    127 / 8) give back q and k; a constant segment and a zero segment have
+    zero remainders; the remainder's sign follows A - s q."""
+    rng = np.random.default_rng(6)
+    q = rng.integers(-126, 127, 63)
+    k = rng.integers(-(2 ** 31) + 1, 2 ** 31 - 1, 63)
+    A = np.zeros((3, 128))
+    A[0, 0] = 127 / 8
+    A[0, 1:64] = q / 8 + k * 2.0 ** -35
+    A[1, :64] = -2.5
+    A[2, 64:] = 0.0
+    A[2, :64] = 1.0
+    A[2, 5] = 1.0 + 2.0 ** -30          # 2^-30 / s * 2^32 = 508 above A[2, 0]'s remainder
+    codes, scales, resid = oi.cast_split(A)
+    assert scales[0, 0] == np.float32(0.125)
+    assert list(codes[0, 1:64].astype(int) - 128) == list(q)
+    assert list(resid[0, 1:64].astype(np.int64)) == list(k)
+    assert resid[0, 0] == 0
+    assert np.all(resid[1, :64] == resid[1, 0]) and np.all(resid[:, 64:] == 0)
+    s = np.float64(scales[2, 0])
+    for c in (0, 5):
+        want = np.rint((A[2, c] - s * 127) / s * 2.0 ** 32)
+        assert resid[2, c] == want
+    assert resid[2, 5] - resid[2, 0] == np.rint(4.0 / s)
+    D = oi.dequantize_split(codes, scales, resid)
+    assert np.array_equal(D[0], A[0])

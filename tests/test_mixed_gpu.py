@@ -11,9 +11,14 @@ the 2,048-triangle box with its dipping fault.
     FP32 vector to FP32 accumulation (1e-5 of the sum of |terms|), and
     holding the near blocks out of A and writing them back leaves A
     bit-identical;
-  * the nested FGMRES (FP64 outer on the exact operator) reaches the FP64
-    true residual 1e-12 and the oracle's LU solution to 1e-9, with and
-    without the recycled space across inner solves; the recycled pairs keep
+  * the split form (codes plus 32-bit rounding remainders) equals the
+    oracle's bit for bit and its FP64 product the oracle's split matrix
+    times x to FP64 rounding (1e-14 of the sum of |terms|), within
+    s 2^-33 per entry of A itself;
+  * the nested FGMRES (FP64 outer on the exact operator, or on the split
+    operator) reaches the FP64 true residual 1e-12 and the oracle's LU
+    solution to 1e-9, with and without the recycled space across inner
+    solves; the recycled pairs keep
     C^T C = I to 1e-10 and op U = C to 1e-5 (the mixed operator accumulates in
     FP32, so re-applying it to combinations of earlier inputs differs at FP32
     rounding).
@@ -58,17 +63,26 @@ def test_mixed_matvec(c1, storage, tol):
     assert err < tol
 
 
-@pytest.mark.parametrize("storage,recycle", [("bf16", 0), ("bf16", 60), ("int8", 60),
-                                             ("int8", 90)])
-def test_mixed_solve(c1, storage, recycle):
+@pytest.mark.parametrize("storage,recycle,outer", [("bf16", 0, "fp64"), ("bf16", 60, "fp64"),
+                                                   ("int8", 60, "fp64"), ("int8", 90, "fp64"),
+                                                   ("int8", 90, "split")])
+def test_mixed_solve(c1, storage, recycle, outer):
+    """The last case is the bench's method: int8 far field, 90 recycled
+    vectors, rigid coarse space and the outer loop on the split operator
+    (int8 codes + 32-bit remainders); its residual is also taken with the
+    oracle's FP64 matrix."""
     from mbem_amd.la.mixed import solve_mixed
     asm = c1["asm"]
-    x, rep = solve_mixed(asm, rtol=1e-12, storage=storage, recycle=recycle)
+    x, rep = solve_mixed(asm, rtol=1e-12, storage=storage, recycle=recycle, outer=outer,
+                         coarse="rigid" if outer == "split" else None)
     xh = x.cpu().numpy()
     err = np.linalg.norm(xh - c1["x_lu"]) / np.linalg.norm(c1["x_lu"])
-    print(f"{rep}; rel err vs oracle LU {err:.2e}")
+    res64 = np.linalg.norm(c1["b"] - c1["A"] @ xh) / np.linalg.norm(c1["b"])
+    print(f"{rep}; rel err vs oracle LU {err:.2e}, oracle FP64 residual {res64:.2e}")
     assert rep.converged and rep.outer.true_relres < 1e-12
+    assert rep.outer_operator == outer
     assert err < 1e-9
+    assert res64 < 2e-12
 
 
 @pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030), (45, 5000),
@@ -114,6 +128,58 @@ def test_int8_cast_and_product_match_oracle(m, n):
     err = np.abs(y.cpu().numpy() - ref) / scale
     print(f"int8 {m}x{n}: product err / |terms| max {err.max():.2e}")
     assert err.max() < 1e-5
+
+
+@pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030), (45, 5000),
+                                 (3, 8208), (13, 4099), (7, 2050)])
+def test_int8_split_cast_and_product_match_oracle(m, n):
+    """The split form (codes, scales and 32-bit remainders) bit for bit
+    against the oracle, and its FP64 product against the oracle's split
+    matrix times x: 2500 columns leave one LDS tile (2048) and tail groups of
+    four, 5000 two tiles with waves past the last row, 8208 four tiles and
+    four tail groups, 4099 two tiles and a 3-column remainder, 2050 one tile
+    and a 2-column remainder, 1030 and 16 no tile at all."""
+    import torch
+    from mbem_amd._lib import call, load, ptr, stream
+    from oracle import int8 as oi
+    rng = np.random.default_rng(3 * m + n)
+    A = rng.standard_normal((m, n)) * np.exp(rng.uniform(-20, 5, n))[None, :]
+    A[m // 2] *= 1e-11
+    A[0, :64] = 0.0
+    dev = torch.device("cuda")
+    lda = n + 3
+    Ad = torch.zeros((m, lda), dtype=torch.float64, device=dev)
+    Ad[:, :n] = torch.as_tensor(A, device=dev)
+    nseg = int(load().mbem_i8_segments(n))
+    ldf = ((n + 15) // 16) * 16
+    ldr = ((n + 3) // 4) * 4 + 4
+    F = torch.zeros((m, ldf), dtype=torch.uint8, device=dev)
+    S = torch.zeros((m, nseg), dtype=torch.float32, device=dev)
+    E = torch.zeros((m, ldr), dtype=torch.int32, device=dev)
+    call("mbem_cast_i8e", ptr(Ad), lda, m, n, ptr(F), ldf, ptr(S), nseg, ptr(E), ldr, stream())
+    codes, scales, resid = oi.cast_split(A)
+    assert np.array_equal(F[:, :n].cpu().numpy(), codes)
+    assert np.array_equal(S.cpu().numpy(), scales)
+    assert np.array_equal(E[:, :n].cpu().numpy(), resid)
+    x = rng.standard_normal(n)
+    xd = torch.as_tensor(x, device=dev)
+    y0 = rng.standard_normal(m)
+    y = torch.as_tensor(y0, device=dev)
+    call("mbem_gemv_i8e", ptr(F), ldf, ptr(E), ldr, ptr(S), nseg, m, n, ptr(xd), ptr(y), -0.5, 2.0,
+         stream())
+    yh = y.cpu().numpy()
+    D = oi.dequantize_split(codes, scales, resid)
+    ref = -0.5 * (D @ x) + 2.0 * y0
+    # the kernel sums (u x - 128 x) + 2^-32 e x per four columns in FP64: the
+    # bar is FP64 rounding of terms up to 256 s |x|
+    step = scales[:, np.arange(n) // 64].astype(np.float64)
+    scale = 0.5 * ((256.0 * step) @ np.abs(x)) + 2.0 * np.abs(y0)
+    err = np.abs(yh - ref) / scale
+    print(f"split {m}x{n}: product err / |terms| max {err.max():.2e}")
+    assert err.max() < 1e-14
+    # and against A itself: the split form is A to within s 2^-33 per entry
+    bound = 0.5 * ((step * 2.0 ** -33 + np.abs(A) * 2.0 ** -52) @ np.abs(x)) + 1e-14 * scale
+    assert np.all(np.abs(yh - (-0.5 * (A @ x) + 2.0 * y0)) <= bound * 1.01)
 
 
 def test_near_hold_round_trip(c1):
