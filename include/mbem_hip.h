@@ -327,6 +327,31 @@ int mbem_gemv_i8e(const uint8_t *F, int64_t ldf, const int32_t *resid, int64_t l
 int mbem_bsr_gemv(int64_t nbr, const int64_t *rowptr, const int64_t *col, const double *vals,
                   const double *x, double *y, double alpha, hipStream_t stream);
 
+/* FP32 block-Jacobi chunk inverses, every chunk in one launch sequence
+ * (blocked Gauss-Jordan with partial pivoting; replaces the per-chunk
+ * lu_factor of la/preconditioner.py:448 for the mixed method's FP32
+ * inverses).  desc holds mbem_bj_desc_fields() int64 per chunk: offset of
+ * its DOF list in idx, size m (<= mmax <= 2048), working matrix offset (a
+ * multiple of 4) and row pitch (a multiple of mbem_bj_panel_width(), >= m),
+ * output offset and pitch (>= m), row-map offset, saved-rows offset (a
+ * multiple of 4; panel width x pitch floats).  desc_host is the same table in
+ * host memory: it is checked against the lengths given (elements) before
+ * anything is launched.  Gather: W_c = (float) A[idx_c, idx_c], the row map
+ * reset, info zeroed (a caller changing A may do so once this is queued).
+ * Invert: out_c (row-major, pads zero) = W_c^-1; info[c] = 1 + the first
+ * pivot column found zero (or NaN), 0 when none; W, rowmap, piv and saved
+ * are working memory. */
+int mbem_bj_desc_fields(void);
+int mbem_bj_panel_width(void);
+int mbem_bj_gather_f32(const double *A, int64_t lda, const int64_t *idx, int64_t idx_len,
+                       const int64_t *desc_host, const int64_t *desc, int64_t nchunks,
+                       int64_t mmax, float *W, int64_t w_len, int32_t *rowmap, int64_t map_len,
+                       int32_t *info, hipStream_t stream);
+int mbem_bj_invert_f32(const int64_t *desc_host, const int64_t *desc, int64_t nchunks,
+                       int64_t mmax, float *W, int64_t w_len, int32_t *rowmap, int32_t *piv,
+                       int64_t map_len, float *saved, int64_t saved_len, int32_t *info,
+                       float *out, int64_t out_len, hipStream_t stream);
+
 /* FP32 block-Jacobi: chunk gather, strided-batched sgetrf + sgetri, apply. */
 int mbem_gather_block_f32(const double *A, int64_t lda, const int64_t *idx, int64_t m, float *M,
                           hipStream_t stream);

@@ -83,6 +83,11 @@ _SIGS = {
     "mbem_bj_apply_f32": [_p, _i64, _p, _p, _p, _p, _p, _p, _p],
     "mbem_gather_f32": [_p, _i64, _p, _p, _p],
     "mbem_bj_apply_f32v": [_p, _i64, _p, _p, _p, _p, _p, _p, _p, _p],
+    "mbem_bj_desc_fields": [],
+    "mbem_bj_panel_width": [],
+    "mbem_bj_gather_f32": [_p, _i64, _p, _i64, _p, _p, _i64, _i64, _p, _i64, _p, _i64, _p, _p],
+    "mbem_bj_invert_f32": [_p, _p, _i64, _i64, _p, _i64, _p, _p, _i64, _p, _i64, _p, _p, _i64,
+                           _p],
     "mbem_src_fields": [],
     "mbem_multidot_work": [_i64, _i64],
     # batched Laplace-state solves

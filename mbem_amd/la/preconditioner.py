@@ -23,6 +23,7 @@ outer iteration absorbs (the outer residual is the true one).
 """
 from __future__ import annotations
 
+import ctypes
 import time
 import weakref
 
@@ -36,6 +37,9 @@ from .cluster import build_cluster_tree, leaves
 
 F64 = torch.float64
 I64 = torch.int64
+# largest FP32 chunk the batched Gauss-Jordan inverts (four row slots of its
+# 512-thread workgroup); larger chunks go to rocSOLVER
+BJ_GJ_MAX_DOF = 2048
 
 
 def group_slots(model, layout, grouping: str) -> list:
@@ -210,6 +214,10 @@ class BlockGaussSeidel:
         self.chunk_ptr = _lib.upload(cptr, I64)
         self.inv_ptr = _lib.upload(iptr, I64)
         self.idx = _lib.upload(np.concatenate([c[1] for c in chunks]), I64)
+        # FP32 chunks up to BJ_GJ_MAX_DOF are inverted by the project's batched
+        # Gauss-Jordan (every element of the packed inverse written), larger
+        # ones by rocSOLVER per size class
+        use_gj = f32 and sizes.size > 0 and int(sizes.max()) <= BJ_GJ_MAX_DOF
         if f32:
             pptr = np.concatenate([[0], np.cumsum(ld)])
             gidx = np.full(int(pptr[-1]), -1, dtype=np.int64)
@@ -218,7 +226,8 @@ class BlockGaussSeidel:
             self.pad_ptr = _lib.upload(pptr, I64)
             self.gidx = _lib.upload(gidx, I64)
             self.rr32 = torch.zeros(int(pptr[-1]), dtype=torch.float32, device=dev)
-            self.inv = torch.zeros(int(iptr[-1]), dtype=torch.float32, device=dev)
+            self.inv = (torch.empty if use_gj else torch.zeros)(int(iptr[-1]),
+                                                                dtype=torch.float32, device=dev)
         else:
             self.inv = torch.empty(int(iptr[-1]), dtype=F64, device=dev)
         self._ld = ld
@@ -235,57 +244,61 @@ class BlockGaussSeidel:
                 sb.pad = (int(pptr[c0]), int(pptr[c1]))
         for ci, (k, dofs) in enumerate(chunks):
             tables[k] += [(ci, r0) for r0 in range(0, int(dofs.size), R)]
-        # invert the chunks in size classes (largest / smallest <= 1.1): each
-        # class is padded to its largest size with an identity block and
-        # inverted in one strided-batched getrf + getri, then the leading
-        # blocks are packed into ``inv``.  Every class is gathered first, and
-        # ``gathered`` (an event on the stream) marks the last read of A, so a
-        # caller may change A on another stream after it while the inversions
-        # run on the padded copies.
-        order = np.argsort(sizes, kind="stable")
-        esz = 4 if f32 else 8
-        classes = []
-        c0 = 0
-        while c0 < len(order):
-            c1 = c0
-            while c1 + 1 < len(order) and sizes[order[c1 + 1]] <= 1.1 * sizes[order[c0]]:
-                c1 += 1
-            cls = order[c0:c1 + 1]
-            mp = int(sizes[cls].max())
-            tmp = torch.empty((len(cls), mp, mp), dtype=torch.float32 if f32 else F64, device=dev)
-            for q, ci in enumerate(cls):
-                call("mbem_gather_block_pad", ptr(A), A.stride(0),
-                     ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), int(sizes[ci]), mp,
-                     ptr(tmp[q]), int(f32), stream())
-            classes.append((cls, mp, tmp))
-            c0 = c1 + 1
-        # host copies of the chunk DOF lists (a coarse space is built on them;
-        # ``after_gather`` receives them with the event)
-        self.chunk_dofs = [c[1] for c in chunks]
-        self.gathered = torch.cuda.Event()
-        self.gathered.record()
-        if after_gather is not None:
-            # the caller's own launches (e.g. on another stream, after
-            # ``gathered``) go ahead of the inversions' long launch sequence
-            after_gather(self.gathered, self.chunk_dofs)
-        # getrf's info flags, read back once after every class is queued
-        # (``finish``), so no class waits on the one before.  The inverse of
-        # each class goes to a second buffer (getrf, then A C = I solved with
-        # the factors), then its leading blocks into ``inv``.
-        self._pending = []
-        for cls, mp, tmp in classes:
-            nb = len(cls)
-            ipiv = torch.empty(nb * mp, dtype=torch.int32, device=dev)
-            info = torch.zeros(nb, dtype=torch.int32, device=dev)
-            out = torch.empty_like(tmp)
-            call("mbem_invert_batched_oop", ptr(tmp), mp, nb, ptr(ipiv), ptr(info), ptr(out),
-                 int(f32), stream())
-            self._pending.append((info, cls, sizes[cls]))
-            for q, ci in enumerate(cls):
-                m = int(sizes[ci])
-                call("mbem_copy2d", ptr(out[q]), mp, ptr(self.inv[int(iptr[ci]):]), int(ld[ci]), m,
-                     m, esz, stream())
-        del classes
+        if use_gj:
+            self._pending = self._invert_gj(A, chunks, sizes, cptr, iptr, ld, after_gather)
+        else:
+            # invert the chunks in size classes (largest / smallest <= 1.1): each
+            # class is padded to its largest size with an identity block and
+            # inverted in one strided-batched getrf + getri, then the leading
+            # blocks are packed into ``inv``.  Every class is gathered first, and
+            # ``gathered`` (an event on the stream) marks the last read of A, so a
+            # caller may change A on another stream after it while the inversions
+            # run on the padded copies.
+            order = np.argsort(sizes, kind="stable")
+            esz = 4 if f32 else 8
+            classes = []
+            c0 = 0
+            while c0 < len(order):
+                c1 = c0
+                while c1 + 1 < len(order) and sizes[order[c1 + 1]] <= 1.1 * sizes[order[c0]]:
+                    c1 += 1
+                cls = order[c0:c1 + 1]
+                mp = int(sizes[cls].max())
+                tmp = torch.empty((len(cls), mp, mp), dtype=torch.float32 if f32 else F64,
+                                  device=dev)
+                for q, ci in enumerate(cls):
+                    call("mbem_gather_block_pad", ptr(A), A.stride(0),
+                         ptr(self.idx[int(cptr[ci]):int(cptr[ci + 1])]), int(sizes[ci]), mp,
+                         ptr(tmp[q]), int(f32), stream())
+                classes.append((cls, mp, tmp))
+                c0 = c1 + 1
+            # host copies of the chunk DOF lists (a coarse space is built on them;
+            # ``after_gather`` receives them with the event)
+            self.chunk_dofs = [c[1] for c in chunks]
+            self.gathered = torch.cuda.Event()
+            self.gathered.record()
+            if after_gather is not None:
+                # the caller's own launches (e.g. on another stream, after
+                # ``gathered``) go ahead of the inversions' long launch sequence
+                after_gather(self.gathered, self.chunk_dofs)
+            # getrf's info flags, read back once after every class is queued
+            # (``finish``), so no class waits on the one before.  The inverse of
+            # each class goes to a second buffer (getrf, then A C = I solved with
+            # the factors), then its leading blocks into ``inv``.
+            self._pending = []
+            for cls, mp, tmp in classes:
+                nb = len(cls)
+                ipiv = torch.empty(nb * mp, dtype=torch.int32, device=dev)
+                info = torch.zeros(nb, dtype=torch.int32, device=dev)
+                out = torch.empty_like(tmp)
+                call("mbem_invert_batched_oop", ptr(tmp), mp, nb, ptr(ipiv), ptr(info), ptr(out),
+                     int(f32), stream())
+                self._pending.append((info, cls, sizes[cls]))
+                for q, ci in enumerate(cls):
+                    m = int(sizes[ci])
+                    call("mbem_copy2d", ptr(out[q]), mp, ptr(self.inv[int(iptr[ci]):]),
+                         int(ld[ci]), m, m, esz, stream())
+            del classes
         for k, sb in enumerate(self.sbs):
             wt = np.asarray(tables[k], dtype=np.int64).reshape(-1, 2)
             sb.wave_table = _lib.upload(wt, I64)
@@ -297,6 +310,52 @@ class BlockGaussSeidel:
         self.n_chunks = len(chunks)
         self.inverse_bytes = int(iptr[-1]) * (4 if f32 else 8)     # padding included
         self._rr = torch.empty(n, dtype=F64, device=dev)
+
+    def _invert_gj(self, A, chunks, sizes, cptr, iptr, ld, after_gather) -> list:
+        """Every chunk's FP32 inverse into ``inv`` in one launch sequence: the
+        chunks gathered from A (then ``gathered`` and the ``after_gather``
+        hook, as for the size classes), then the batched blocked Gauss-Jordan
+        of mbem_bj_invert_f32 (partial pivoting by a row map, 32-column
+        panels, two launches per panel for all chunks).  Returns the pending
+        status flags for ``finish``."""
+        lib = _lib.load()
+        nf, B = int(lib.mbem_bj_desc_fields()), int(lib.mbem_bj_panel_width())
+        nc = int(sizes.size)
+        mmax = int(sizes.max())
+        ldw = ((sizes + B - 1) // B) * B
+        woff = np.concatenate([[0], np.cumsum(sizes * ldw)])
+        soff = np.concatenate([[0], np.cumsum(B * ldw)])
+        desc = np.zeros((nc, nf), dtype=np.int64)
+        desc[:, 0] = cptr[:-1]          # the chunk's DOFs in idx
+        desc[:, 1] = sizes
+        desc[:, 2] = woff[:-1]          # working copy, pitch a multiple of the panel width
+        desc[:, 3] = ldw
+        desc[:, 4] = iptr[:-1]          # the packed inverse (pitch ld)
+        desc[:, 5] = ld
+        desc[:, 6] = cptr[:-1]          # row map and pivots
+        desc[:, 7] = soff[:-1]          # saved pivot rows
+        dev = A.device
+        d_dev = _lib.upload(desc, I64)
+        W = torch.empty(int(woff[-1]), dtype=torch.float32, device=dev)
+        rowmap = torch.empty(int(cptr[-1]), dtype=torch.int32, device=dev)
+        piv = torch.empty_like(rowmap)
+        saved = torch.empty(int(soff[-1]), dtype=torch.float32, device=dev)
+        info = torch.empty(nc, dtype=torch.int32, device=dev)
+        dh = ctypes.c_void_p(desc.ctypes.data)
+        call("mbem_bj_gather_f32", ptr(A), A.stride(0), ptr(self.idx), int(cptr[-1]), dh,
+             ptr(d_dev), nc, mmax, ptr(W), W.numel(), ptr(rowmap), rowmap.numel(), ptr(info),
+             stream())
+        self.chunk_dofs = [c[1] for c in chunks]
+        self.gathered = torch.cuda.Event()
+        self.gathered.record()
+        if after_gather is not None:
+            after_gather(self.gathered, self.chunk_dofs)
+        call("mbem_bj_invert_f32", dh, ptr(d_dev), nc, mmax, ptr(W), W.numel(), ptr(rowmap),
+             ptr(piv), rowmap.numel(), ptr(saved), saved.numel(), ptr(info), ptr(self.inv),
+             self.inv.numel(), stream())
+        # the working memory stays referenced until ``finish`` has waited
+        self._gj_work = (W, rowmap, piv, saved, d_dev)
+        return [(info, np.arange(nc), sizes)]
 
     def finish(self) -> "BlockGaussSeidel":
         """Wait for the chunk inversions and raise ``LinAlgError`` for the first
@@ -314,9 +373,10 @@ class BlockGaussSeidel:
                     q = int(np.flatnonzero(bad)[0])
                     raise np.linalg.LinAlgError(f"block-Jacobi chunk {int(cls[q])} "
                                                 f"({int(sz[q])} DOFs) is singular "
-                                                f"(getrf info {int(bad[q])})")
+                                                f"(zero pivot, info {int(bad[q])})")
             self._pending = []
         torch.cuda.current_stream().synchronize()
+        self._gj_work = None
         if self.build_s is None:
             self.build_s = time.perf_counter() - self._t0
         return self

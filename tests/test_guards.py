@@ -117,3 +117,30 @@ def test_split_outer_option_is_checked():
         solve_mixed(None, storage="bf16", outer="split")
     with pytest.raises(ValueError, match="outer"):
         solve_mixed(None, storage="int8", outer="fp32")
+
+
+def test_bj_inverse_descriptors_are_checked_on_the_host():
+    """The chunk inversion's descriptor table (host copy) is checked before
+    any pointer is used: a pitch that is not a multiple of the panel width, a
+    chunk past mmax, or a working matrix past the length given are refused."""
+    from mbem_amd import _lib
+    lib = _lib.load()
+    null = ctypes.c_void_p(0)
+    nf, B = lib.mbem_bj_desc_fields(), lib.mbem_bj_panel_width()
+    good = np.array([[0, 40, 0, B * 2, 0, 40, 0, 0]], dtype=np.int64)
+    assert good.shape[1] == nf
+
+    def gather(desc, mmax=64, w_len=40 * 2 * B):
+        dh = ctypes.c_void_p(desc.ctypes.data)
+        return lib.mbem_bj_gather_f32(null, 100, null, 40, dh, null, 1, mmax, null, w_len, null,
+                                      40, null, null)
+
+    bad_pitch = good.copy()
+    bad_pitch[0, 3] = B * 2 + 1
+    assert gather(bad_pitch) == _lib.MBEM_EINVAL
+    assert gather(good, mmax=39) == _lib.MBEM_EINVAL
+    assert gather(good, w_len=40 * 2 * B - 1) == _lib.MBEM_EINVAL
+    # a valid table with no device pointers is refused too (null matrix)
+    assert gather(good) == _lib.MBEM_EINVAL
+    assert lib.mbem_bj_gather_f32(null, 1, null, 0, null, null, 0, 0, null, 0, null, 0, null,
+                                  null) == _lib.MBEM_OK
