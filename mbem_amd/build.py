@@ -28,11 +28,8 @@ HIPCC_FLAGS = [
 
 
 # per-source extra flags: the nodal kernels' cancellation-sensitive
-# differences round as written (no fused multiply-add contraction); the
-# chunk inversion's 32 pivot steps unroll fully (its register-held panel is
-# indexed by the step: a partly unrolled loop would move it to scratch)
-FILE_FLAGS = {"nodal.hip": ["-ffp-contract=off"],
-              "bjinv.hip": ["-mllvm", "-pragma-unroll-threshold=65536"]}
+# differences round as written (no fused multiply-add contraction)
+FILE_FLAGS = {"nodal.hip": ["-ffp-contract=off"]}
 
 
 def hipcc() -> str:

@@ -126,79 +126,92 @@ __global__ __launch_bounds__(kNT) void k_gj_panel(const int64_t* __restrict__ de
         }
     }
 
-#pragma unroll
+    // Step t works on register column 0: after every step each row's
+    // registers rotate left by one (the step's column to the end), so the
+    // loop body is the same for every step -- one copy of the code, static
+    // register indexes -- and after kB rotations every column is back in
+    // place.  (Unrolled, the 32 steps' code missed the instruction cache:
+    // 11 us a step against about 1.)
+#pragma unroll 1
     for (int t = 0; t < kB; ++t) {
-      // bw is the same for the whole workgroup (the barriers inside are reached by all)
-      if (t < bw) {
-        const int par = t & 1;               // LDS buffers alternate between steps
-        const long kk = k + t;
-        // pivot: this thread's rows, the wave, then the eight waves' results
-        float bv = -1.0f;
-        int bi = 0x7fffffff;
+        // bw is the same for the whole workgroup (the barriers inside are reached by all)
+        if (t < bw) {
+            const int par = t & 1;           // LDS buffers alternate between steps
+            const long kk = k + t;
+            // pivot: this thread's rows, the wave, then the eight waves' results
+            float bv = -1.0f;
+            int bi = 0x7fffffff;
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            const long i = tid + (long)kNT * s;
-            if (i < m && i >= kk) arg_better(bv, bi, fabsf(P[s][t]), (int)i);
-        }
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1)
-            arg_better(bv, bi, __shfl_xor(bv, off, 64), __shfl_xor(bi, off, 64));
-        if (lane == 0) {
-            red_v[par][wave] = bv;
-            red_i[par][wave] = bi;
-        }
-        __syncthreads();
-        float pv = red_v[par][0];
-        int p = red_i[par][0];
-#pragma unroll
-        for (int w = 1; w < NW; ++w) arg_better(pv, p, red_v[par][w], red_i[par][w]);
-        // no candidate compared (all NaN): row kk itself, flagged like a zero pivot
-        const bool bad = (p < kk || p >= m);
-        if (bad) p = (int)kk;
-        // the pivot row's owner publishes it scaled; the owner of row kk its old row
-#pragma unroll
-        for (int s = 0; s < R; ++s) {
-            const long i = tid + (long)kNT * s;
-            if (i == p) {
-                const float a = P[s][t];
-                float r = 0.0f;
-                if (a != 0.0f && !bad) r = 1.0f / a;
-                else atomicCAS(info + blockIdx.x, 0, (int)(kk + 1));
-#pragma unroll
-                for (int c = 0; c < kB; ++c) zs[par][c] = (c == t) ? r : P[s][c] * r;
-                zr[par] = rm[s];
+            for (int s = 0; s < R; ++s) {
+                const long i = tid + (long)kNT * s;
+                if (i < m && i >= kk) arg_better(bv, bi, fabsf(P[s][0]), (int)i);
             }
-            if (i == kk && p != kk) {
 #pragma unroll
-                for (int c = 0; c < kB; ++c) ys[par][c] = P[s][c];
-                yr[par] = rm[s];
+            for (int off = 32; off >= 1; off >>= 1)
+                arg_better(bv, bi, __shfl_xor(bv, off, 64), __shfl_xor(bi, off, 64));
+            if (lane == 0) {
+                red_v[par][wave] = bv;
+                red_i[par][wave] = bi;
             }
-        }
-        if (tid == 0) piv[d[D_MAP] + kk] = p;
-        __syncthreads();
-        const float r = zs[par][t];
+            __syncthreads();
+            float pv = red_v[par][0];
+            int p = red_i[par][0];
 #pragma unroll
-        for (int s = 0; s < R; ++s) {
-            const long i = tid + (long)kNT * s;
-            if (i >= m) continue;
-            if (i == kk) {
+            for (int w = 1; w < NW; ++w) arg_better(pv, p, red_v[par][w], red_i[par][w]);
+            // no candidate compared (all NaN): row kk itself, flagged like a zero pivot
+            const bool bad = (p < kk || p >= m);
+            if (bad) p = (int)kk;
+            // the pivot row's owner publishes it scaled; the owner of row kk its old row
 #pragma unroll
-                for (int c = 0; c < kB; ++c) P[s][c] = zs[par][c];
-                rm[s] = zr[par];
-            } else {
-                if (i == p) {                // p != kk here: old row kk moves to p
+            for (int s = 0; s < R; ++s) {
+                const long i = tid + (long)kNT * s;
+                if (i == p) {
+                    const float a = P[s][0];
+                    float r = 0.0f;
+                    if (a != 0.0f && !bad) r = 1.0f / a;
+                    else atomicCAS(info + blockIdx.x, 0, (int)(kk + 1));
+                    zs[par][0] = r;
 #pragma unroll
-                    for (int c = 0; c < kB; ++c) P[s][c] = ys[par][c];
-                    rm[s] = yr[par];
+                    for (int c = 1; c < kB; ++c) zs[par][c] = P[s][c] * r;
+                    zr[par] = rm[s];
                 }
-                const float f = P[s][t];
+                if (i == kk && p != kk) {
 #pragma unroll
-                for (int c = 0; c < kB; ++c)
-                    if (c != t) P[s][c] -= f * zs[par][c];
-                P[s][t] = -f * r;
+                    for (int c = 0; c < kB; ++c) ys[par][c] = P[s][c];
+                    yr[par] = rm[s];
+                }
+            }
+            if (tid == 0) piv[d[D_MAP] + kk] = p;
+            __syncthreads();
+            const float r = zs[par][0];
+#pragma unroll
+            for (int s = 0; s < R; ++s) {
+                const long i = tid + (long)kNT * s;
+                if (i >= m) continue;
+                if (i == kk) {
+#pragma unroll
+                    for (int c = 0; c < kB; ++c) P[s][c] = zs[par][c];
+                    rm[s] = zr[par];
+                } else {
+                    if (i == p) {            // p != kk here: old row kk moves to p
+#pragma unroll
+                        for (int c = 0; c < kB; ++c) P[s][c] = ys[par][c];
+                        rm[s] = yr[par];
+                    }
+                    const float f = P[s][0];
+#pragma unroll
+                    for (int c = 1; c < kB; ++c) P[s][c] -= f * zs[par][c];
+                    P[s][0] = -f * r;
+                }
             }
         }
-      }
+#pragma unroll
+        for (int s = 0; s < R; ++s) {
+            const float first = P[s][0];
+#pragma unroll
+            for (int c = 0; c < kB - 1; ++c) P[s][c] = P[s][c + 1];
+            P[s][kB - 1] = first;
+        }
     }
 
     // the panel (now the steps' transform) and the row map back; the pivot

@@ -313,11 +313,11 @@ class BlockGaussSeidel:
 
     def _invert_gj(self, A, chunks, sizes, cptr, iptr, ld, after_gather) -> list:
         """Every chunk's FP32 inverse into ``inv`` in one launch sequence: the
-        chunks gathered from A (then ``gathered`` and the ``after_gather``
-        hook, as for the size classes), then the batched blocked Gauss-Jordan
-        of mbem_bj_invert_f32 (partial pivoting by a row map, 32-column
-        panels, two launches per panel for all chunks).  Returns the pending
-        status flags for ``finish``."""
+        chunks gathered from A (then ``gathered``, as for the size classes),
+        the batched blocked Gauss-Jordan of mbem_bj_invert_f32 queued
+        (partial pivoting by a row map, 32-column panels, two launches per
+        panel for all chunks), then the ``after_gather`` hook.  Returns the
+        pending status flags for ``finish``."""
         lib = _lib.load()
         nf, B = int(lib.mbem_bj_desc_fields()), int(lib.mbem_bj_panel_width())
         nc = int(sizes.size)
@@ -348,11 +348,13 @@ class BlockGaussSeidel:
         self.chunk_dofs = [c[1] for c in chunks]
         self.gathered = torch.cuda.Event()
         self.gathered.record()
-        if after_gather is not None:
-            after_gather(self.gathered, self.chunk_dofs)
+        # the inversion's few launches go ahead of the hook's host work (which
+        # waits for the near pattern and queues the caller's side-stream build)
         call("mbem_bj_invert_f32", dh, ptr(d_dev), nc, mmax, ptr(W), W.numel(), ptr(rowmap),
              ptr(piv), rowmap.numel(), ptr(saved), saved.numel(), ptr(info), ptr(self.inv),
              self.inv.numel(), stream())
+        if after_gather is not None:
+            after_gather(self.gathered, self.chunk_dofs)
         # the working memory stays referenced until ``finish`` has waited
         self._gj_work = (W, rowmap, piv, saved, d_dev)
         return [(info, np.arange(nc), sizes)]
