@@ -6,8 +6,8 @@
 // The DOFs are listed group by group in ``perm`` (gptr[g] .. gptr[g+1]) and
 // Wp[t q + j] is the j-th value at the t-th listed DOF.  The kernels:
 //   * the coarse image Ct = (A W)^T (K x n, row-major): one pass over A, a
-//     workgroup per 4 R rows, the group's listed DOFs staged through LDS
-//     for all of them (below);
+//     workgroup per (group, 128 rows), the group's listed DOFs staged
+//     through LDS for all of its rows (below);
 //   * the same image of the mixed operator (int8 far field plus the FP64
 //     near field's block-CSR: 8x fewer bytes than the FP64 pass);
 //   * the expansion x[perm[t]] += alpha sum_j Wp[t q + j] u[grp[t] q + j],
@@ -30,19 +30,26 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 // Ct[(g q + j) ldc + r] = sum_{t in group g} A[r, perm[t]] Wp[t q + j].
-// A workgroup of four waves takes 4 R consecutive rows, a wave R of them.
-// The group's listed positions go through LDS 256 at a time (their column
-// and their q shape values, loaded once for the workgroup's 4 R rows rather
-// than once per wave); each wave then reads its R rows at those columns
-// (ascending within a group, so the gathered reads walk each row forward in
-// runs), keeps q partial sums per row and reduces them across the wave once
-// per group.  Rows past m are clamped to m - 1 and not stored; every thread
+// A workgroup takes one group g and one block of 4 R P consecutive rows;
+// each of its four waves takes P passes of R rows.  Workgroups go to the
+// eight XCDs in turn (block b to XCD b mod 8), so the block index is mapped
+// for each XCD to run one row block's groups one after another: the groups
+// of a row block read the same rows of the matrix (each in short runs), and
+// these reads then meet in that XCD's L2 instead of each fetching its rows'
+// lines from HBM again.  In a pass
+// the group's listed positions go through LDS 256 at a time (their column
+// and their q shape values, loaded once for the workgroup's four waves);
+// each wave reads its R rows at those columns (ascending within a group, so
+// the gathered reads walk each row forward in runs), keeps q partial sums
+// per row in registers over the group's tiles and reduces them across the
+// wave once per pass.  The group loop lives in the grid: each workgroup's
+// chain of dependent tile loads is the group's few tiles times P, not every
+// group's.  Rows past m are clamped to m - 1 and not stored; every thread
 // reaches every barrier (the only exit is the whole workgroup's).  ROW loads
-// one matrix entry as a double: FP64 rows, or int8 offset-binary codes
-// times their row's FP32 scale per 64 columns.
+// one matrix entry as a double (the FP64 rows of A).
 constexpr int kTile = 256;
 
-template <int R, int Q, class ROW>
+template <int R, int Q, int P, class ROW>
 __device__ __forceinline__ void coarse_image_wg(const ROW& row, long m,
                                                 const int64_t* __restrict__ perm,
                                                 const double* __restrict__ Wp,
@@ -52,14 +59,17 @@ __device__ __forceinline__ void coarse_image_wg(const ROW& row, long m,
     __shared__ long s_col[kTile];
     __shared__ double s_w[kTile * Q];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const long wg0 = (long)blockIdx.x * 4 * R;
+    const long b = blockIdx.x, local = b >> 3;
+    const long g = local % G;
+    const long rb = (b & 7) + 8 * (local / G);
+    const long wg0 = rb * 4 * R * P;
     if (wg0 >= m) return;                       // the whole workgroup
-    const long r0 = wg0 + (long)wave * R;
-    long rr[R];
+    const long t0 = gptr[g], t1 = gptr[g + 1];
+    for (int pass = 0; pass < P; ++pass) {
+        const long r0 = wg0 + ((long)wave * P + pass) * R;
+        long rr[R];
 #pragma unroll
-    for (int r = 0; r < R; ++r) rr[r] = (r0 + r < m) ? r0 + r : m - 1;
-    for (long g = 0; g < G; ++g) {
-        const long t0 = gptr[g], t1 = gptr[g + 1];
+        for (int r = 0; r < R; ++r) rr[r] = (r0 + r < m) ? r0 + r : m - 1;
         double acc[R][Q];
 #pragma unroll
         for (int r = 0; r < R; ++r)
@@ -69,7 +79,13 @@ __device__ __forceinline__ void coarse_image_wg(const ROW& row, long m,
             const long nt = (t1 - tb < kTile) ? t1 - tb : kTile;
             __syncthreads();                    // the last tile's readers are done
             if (threadIdx.x < nt) s_col[threadIdx.x] = perm[tb + threadIdx.x];
-            for (long e = threadIdx.x; e < nt * Q; e += blockDim.x) s_w[e] = Wp[tb * Q + e];
+            // shape values stored value-major (s_w[j][u]): a wave's reads of
+            // one value are then lane-contiguous (position-major, 8 Q bytes
+            // apart, they met in a quarter of the LDS banks)
+            for (long e = threadIdx.x; e < nt * Q; e += blockDim.x) {
+                const long u = e / Q, j = e - u * Q;
+                s_w[j * kTile + u] = Wp[tb * Q + e];
+            }
             __syncthreads();
             for (long u = lane; u < nt; u += 64) {
                 const long col = s_col[u];
@@ -78,7 +94,7 @@ __device__ __forceinline__ void coarse_image_wg(const ROW& row, long m,
                 for (int r = 0; r < R; ++r) a[r] = row(rr[r], col);
 #pragma unroll
                 for (int j = 0; j < Q; ++j) {
-                    const double w = s_w[u * Q + j];
+                    const double w = s_w[j * kTile + u];
 #pragma unroll
                     for (int r = 0; r < R; ++r) acc[r][j] += a[r] * w;
                 }
@@ -101,30 +117,30 @@ struct RowF64 {
     __device__ double operator()(long r, long c) const { return A[r * lda + c]; }
 };
 
-struct RowI8 {
-    const uint8_t* F;
-    long ldf;
-    const float* S;
-    long nseg;
-    __device__ double operator()(long r, long c) const
-    {
-        return (double)((float)((int)F[r * ldf + c] - 128) * S[r * nseg + (c >> 6)]);
-    }
-};
-
-template <int R, int Q>
+template <int R, int Q, int P>
 __global__ __launch_bounds__(256) void k_coarse_image(const double* __restrict__ A, long lda, long m,
                                                       const int64_t* __restrict__ perm,
                                                       const double* __restrict__ Wp,
                                                       const int64_t* __restrict__ gptr, long G,
                                                       double* __restrict__ Ct, long ldc)
 {
-    coarse_image_wg<R, Q>(RowF64{A, lda}, m, perm, Wp, gptr, G, Ct, ldc);
+    coarse_image_wg<R, Q, P>(RowF64{A, lda}, m, perm, Wp, gptr, G, Ct, ldc);
 }
 
 // The same image for the mixed operator's far field F: int8 offset-binary
-// codes (value = (code - 128) * scale[r, col / 64]), near entries zero.
-template <int R, int Q>
+// codes (value = (code - 128) * scale[r, col / 64]), near entries zero.  Its
+// own kernel, in FP32: the operator it images is the int8 field (1.8e-4 from
+// A), so FP32 products and sums (1e-7) change nothing that matters, and they
+// take fewer instructions per entry than the FP64 form (27 VALU and 8 LDS
+// instructions an entry measured there, 7.2 ms at n = 60,000): each entry
+// is one FP32 value, the weights are FP32 in LDS, the cross-lane sums FP32,
+// and a wave's row bases are wave-uniform (scalar registers), so a load is
+// that base plus a 32-bit column.  A workgroup holds up to 1,024 of its
+// group's positions in LDS at once (a larger group goes in tiles) and takes
+// 4 waves x P passes x R rows; blocks map to XCDs as in coarse_image_wg.
+constexpr int kTileI8 = 1024;
+
+template <int R, int Q, int P>
 __global__ __launch_bounds__(256) void k_coarse_image_i8(const uint8_t* __restrict__ F, long ldf,
                                                          const float* __restrict__ S, long nseg,
                                                          long m, const int64_t* __restrict__ perm,
@@ -132,7 +148,67 @@ __global__ __launch_bounds__(256) void k_coarse_image_i8(const uint8_t* __restri
                                                          const int64_t* __restrict__ gptr, long G,
                                                          double* __restrict__ Ct, long ldc)
 {
-    coarse_image_wg<R, Q>(RowI8{F, ldf, S, nseg}, m, perm, Wp, gptr, G, Ct, ldc);
+    __shared__ int s_col[kTileI8];
+    __shared__ float s_w[Q][kTileI8];
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long b = blockIdx.x, local = b >> 3;
+    const long g = local % G;
+    const long rb = (b & 7) + 8 * (local / G);
+    const long wg0 = rb * 4 * R * P;
+    if (wg0 >= m) return;                       // the whole workgroup
+    const long t0 = gptr[g], t1 = gptr[g + 1];
+    for (int pass = 0; pass < P; ++pass) {
+        const long r0 = wg0 + ((long)wave * P + pass) * R;
+        const uint8_t* fr[R];
+        const float* sr[R];
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            const long rr = (r0 + r < m) ? r0 + r : m - 1;
+            fr[r] = F + rr * ldf;
+            sr[r] = S + rr * nseg;
+        }
+        float acc[R][Q];
+#pragma unroll
+        for (int r = 0; r < R; ++r)
+#pragma unroll
+            for (int j = 0; j < Q; ++j) acc[r][j] = 0.0f;
+        for (long tb = t0; tb < t1; tb += kTileI8) {
+            const int nt = (int)((t1 - tb < kTileI8) ? t1 - tb : kTileI8);
+            // a group of at most kTileI8 positions is staged once for all passes
+            if (pass == 0 || t1 - t0 > kTileI8) {
+                __syncthreads();                // the last tile's readers are done
+                for (int u = threadIdx.x; u < nt; u += blockDim.x) s_col[u] = (int)perm[tb + u];
+                for (long e = threadIdx.x; e < (long)nt * Q; e += blockDim.x) {
+                    const long u = e / Q, j = e - u * Q;
+                    s_w[j][u] = (float)Wp[tb * Q + e];
+                }
+                __syncthreads();
+            }
+            for (int u = lane; u < nt; u += 64) {
+                const int col = s_col[u];
+                float a[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) a[r] = ((float)fr[r][col] - 128.0f) * sr[r][col >> 6];
+#pragma unroll
+                for (int j = 0; j < Q; ++j) {
+                    const float w = s_w[j][u];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) acc[r][j] += a[r] * w;
+                }
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+#pragma unroll
+            for (int j = 0; j < Q; ++j) {
+                float v = acc[r][j];
+#pragma unroll
+                for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+                if (lane == 0 && r0 + r < m) Ct[(g * Q + j) * ldc + r0 + r] = (double)v;
+            }
+        }
+    }
 }
 
 // Ct += the image of the near field's FP64 3 x 3 block-CSR (block row i,
@@ -180,14 +256,41 @@ __global__ __launch_bounds__(256) void k_coarse_expand(const double* __restrict_
     x[perm[t]] += alpha * s;
 }
 
-constexpr int kRows = 4;
+constexpr int kRows = 4;       // rows per pass of a wave
+constexpr int kPasses = 8;     // passes per wave: 128 rows per workgroup
+
+// row blocks rounded up to a multiple of the eight XCDs, times the groups
+// (blocks past the last row exit at once)
+inline int64_t image_blocks(int64_t m, int64_t G)
+{
+    const int64_t rows = 4 * kRows * kPasses;
+    const int64_t nrb = (m + rows - 1) / rows;
+    const int64_t b = ((nrb + 7) / 8) * 8 * G;
+    return b > 0x7fffffff ? -1 : b;
+}
+
+inline dim3 image_grid(int64_t m, int64_t G) { return dim3((unsigned)image_blocks(m, G)); }
+
+// the int8 image: 64 rows per workgroup (a row block's int8 rows, 3.8 MB at
+// n = 60,000, fit one XCD's L2 while its groups read them)
+constexpr int kPassesI8 = 4;
+
+inline int64_t image_blocks_i8(int64_t m, int64_t G)
+{
+    const int64_t rows = 4 * kRows * kPassesI8;
+    const int64_t nrb = (m + rows - 1) / rows;
+    const int64_t b = ((nrb + 7) / 8) * 8 * G;
+    return b > 0x7fffffff ? -1 : b;
+}
+
+inline dim3 image_grid_i8(int64_t m, int64_t G) { return dim3((unsigned)image_blocks_i8(m, G)); }
 
 template <int Q>
 int image_q(const double* A, int64_t lda, int64_t m, const int64_t* perm, const double* Wp,
             const int64_t* gptr, int64_t G, double* Ct, int64_t ldc, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_coarse_image<kRows, Q>), dim3((unsigned)((m + 4 * kRows - 1) / (4 * kRows))),
-                       dim3(256), 0, stream, A, lda, m, perm, Wp, gptr, G, Ct, ldc);
+    hipLaunchKernelGGL((k_coarse_image<kRows, Q, kPasses>), image_grid(m, G), dim3(256), 0, stream,
+                       A, lda, m, perm, Wp, gptr, (long)G, Ct, ldc);
     return launch_status();
 }
 
@@ -196,8 +299,8 @@ int image_i8_q(const uint8_t* F, int64_t ldf, const float* S, int64_t nseg, int6
                const int64_t* perm, const double* Wp, const int64_t* gptr, int64_t G, double* Ct,
                int64_t ldc, hipStream_t stream)
 {
-    hipLaunchKernelGGL((k_coarse_image_i8<kRows, Q>), dim3((unsigned)((m + 4 * kRows - 1) / (4 * kRows))),
-                       dim3(256), 0, stream, F, ldf, S, nseg, m, perm, Wp, gptr, G, Ct, ldc);
+    hipLaunchKernelGGL((k_coarse_image_i8<kRows, Q, kPassesI8>), image_grid_i8(m, G), dim3(256), 0,
+                       stream, F, ldf, S, nseg, m, perm, Wp, gptr, (long)G, Ct, ldc);
     return launch_status();
 }
 
@@ -227,6 +330,7 @@ int mbem_coarse_image(const double* A, int64_t lda, int64_t m, const int64_t* pe
                       int64_t ldc, hipStream_t stream)
 {
     if (m < 0 || G < 0 || ldc < m || (q != 3 && q != 6 && q != 12)) return MBEM_EINVAL;
+    if (image_blocks(m, G) < 0) return MBEM_EINVAL;            // past a 32-bit grid
     if (m == 0 || G == 0) return MBEM_OK;
     if (!A || !perm || !Wp || !gptr || !Ct) return MBEM_EINVAL;
     switch (q) {
@@ -241,6 +345,7 @@ int mbem_coarse_image_i8(const uint8_t* F, int64_t ldf, const float* scale, int6
                          double* Ct, int64_t ldc, hipStream_t stream)
 {
     if (m < 0 || G < 0 || ldc < m || nseg < 0 || (q != 3 && q != 6 && q != 12)) return MBEM_EINVAL;
+    if (image_blocks_i8(m, G) < 0 || m > 0x7fffffffL) return MBEM_EINVAL;            // past a 32-bit grid
     if (m == 0 || G == 0) return MBEM_OK;
     if (!F || !scale || !perm || !Wp || !gptr || !Ct) return MBEM_EINVAL;
     switch (q) {
