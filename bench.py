@@ -207,8 +207,9 @@ def cpu_baseline(model, eps, system, n_iters, n_rows=48, gemv_rows=3000):
 def kernel_probe(n: int, reps: int):
     """Traffic probe for the PMC counters: ``reps`` launches of the FGMRES
     matvec kernel on an n x n FP64 matrix, ``reps`` launches of the mixed
-    method's BF16 far-field matvec on its BF16 copy and of the int8 one on
-    its int8 copy, then ``reps`` plain device copies of the FP64 matrix (the
+    method's BF16 far-field matvec on its BF16 copy, of the int8 one on its
+    int8 copy and of the split outer matvec (int8 codes + 32-bit remainders,
+    FP64 x), then ``reps`` plain device copies of the FP64 matrix (the
     counter calibration: a copy of B bytes reads B and writes B).  Prints the
     per-launch algorithmic bytes."""
     import torch
@@ -233,20 +234,26 @@ def kernel_probe(n: int, reps: int):
     nseg = int(load().mbem_i8_segments(n))
     F8 = torch.empty((n, ld8), dtype=torch.uint8, device=dev)
     S8 = torch.empty((n, nseg), dtype=torch.float32, device=dev)
-    call("mbem_cast_i8", ptr(A), n, n, n, ptr(F8), ld8, ptr(S8), nseg, stream())
+    ldr = (n + 3) // 4 * 4
+    E = torch.empty((n, ldr), dtype=torch.int32, device=dev)
+    call("mbem_cast_i8e", ptr(A), n, n, n, ptr(F8), ld8, ptr(S8), nseg, ptr(E), ldr, stream())
     for _ in range(reps):
         call("mbem_gemv_i8", ptr(F8), ld8, ptr(S8), nseg, n, n, ptr(xf), ptr(y), 1.0, 0.0,
              stream())
     torch.cuda.synchronize()
-    del F8, S8
+    for _ in range(reps):
+        call("mbem_gemv_i8e", ptr(F8), ld8, ptr(E), ldr, ptr(S8), nseg, n, n, ptr(x), ptr(y),
+             1.0, 0.0, stream())
+    torch.cuda.synchronize()
+    del F8, S8, E
     B = torch.empty_like(A)
     for _ in range(reps):
         B.copy_(A)
     torch.cuda.synchronize()
-    print(json.dumps({"probe": "gemv+gemv_lo+gemv_i8+copy", "n": n,
+    print(json.dumps({"probe": "gemv+gemv_lo+gemv_i8+gemv_i8e+copy", "n": n,
                       "gemv_bytes": n * n * 8 + 2 * n * 8,
                       "gemv_lo_bytes": n * n * 2 + n * 4 + n * 8,
-                      "gemv_i8_bytes": i8_bytes(n),
+                      "gemv_i8_bytes": i8_bytes(n), "gemv_i8e_bytes": i8e_bytes(n),
                       "copy_bytes_each_way": n * n * 8, "reps": reps}), flush=True)
 
 
