@@ -285,6 +285,7 @@ def _fgmres_device(A_mv, M_mv, b, x0, rtol, restart, maxiter, stagnation_window,
     r = torch.empty(n, dtype=F64, device=dev)
 
     x_is_zero = x0 is None
+    r_is_current = False                             # r holds b - A x for the current x
     while total_iters < maxiter and not stagnated:
         if x_is_zero:
             r.copy_(b)                               # r = b - A 0, no matvec
@@ -293,6 +294,7 @@ def _fgmres_device(A_mv, M_mv, b, x0, rtol, restart, maxiter, stagnation_window,
             A_mv(x, r)
             n_mv += 1
             axpby(1.0, b, -1.0, r)                   # r = b - A x
+        r_is_current = True
         beta = ws.norm(r)
         arnoldi_res = beta / b_norm
         if arnoldi_res < rtol:
@@ -357,13 +359,18 @@ def _fgmres_device(A_mv, M_mv, b, x0, rtol, restart, maxiter, stagnation_window,
             y = _solve_upper(H[:j_done, :j_done], g[:j_done])
             yd = torch.as_tensor(y, dtype=F64, device=dev)
             ws.axpy_rows(Z, j_done, yd, x, 1.0)
+            r_is_current = False
         restarts += 1
         if stagnated:
             break
 
-    A_mv(x, r)
-    n_mv += 1
-    axpby(1.0, b, -1.0, r)
+    # the true residual of the returned x; when the restart loop ended right
+    # after computing it (a converged restart check), that r is reused
+    # instead of a second product with A
+    if not r_is_current:
+        A_mv(x, r)
+        n_mv += 1
+        axpby(1.0, b, -1.0, r)
     true_relres = ws.norm(r) / b_norm
     arnoldi_final = history[-1] if history else true_relres
     converged = true_relres < rtol
