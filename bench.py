@@ -13,8 +13,8 @@ untimed, at rtol 1e-13.  Two methods are stepped: plain FP64 (block-Jacobi
 FGMRES) and mixed (int8 far field + FP64 near field inner FGMRES with FP32
 block-Jacobi, a rigid coarse space and a recycled Krylov space, under an
 FP64 outer FGMRES whose operator is, by default, the int8 codes plus their
-32-bit rounding remainders: every far entry within 1e-12 of its segment's
-largest, from 5 bytes instead of 8); ``value`` is the mixed step, the plain
+24-bit rounding remainders: every far entry within 2e-10 of its segment's
+largest, from 4 bytes instead of 8); ``value`` is the mixed step, the plain
 step and the ratio are in ``config``.
 The learned preconditioner's A/B on the same system is reported beside it.
 
@@ -74,9 +74,9 @@ def i8_bytes(n: int) -> int:
 
 def i8e_bytes(n: int) -> int:
     """Algorithmic bytes of one split far-field matvec (n x n): the int8
-    codes, the 32-bit remainders, one FP32 scale per row and 64 columns, x
+    codes, the 24-bit remainders, one FP32 scale per row and 64 columns, x
     and y in FP64."""
-    return n * n + 4 * n * n + 4 * n * (-(-n // 64)) + 8 * n + 8 * n
+    return n * n + 3 * n * n + 4 * n * (-(-n // 64)) + 8 * n + 8 * n
 
 
 def _release(full: bool = False):
@@ -208,7 +208,7 @@ def kernel_probe(n: int, reps: int):
     """Traffic probe for the PMC counters: ``reps`` launches of the FGMRES
     matvec kernel on an n x n FP64 matrix, ``reps`` launches of the mixed
     method's BF16 far-field matvec on its BF16 copy, of the int8 one on its
-    int8 copy and of the split outer matvec (int8 codes + 32-bit remainders,
+    int8 copy and of the split outer matvec (int8 codes + 24-bit remainders,
     FP64 x), then ``reps`` plain device copies of the FP64 matrix (the
     counter calibration: a copy of B bytes reads B and writes B).  Prints the
     per-launch algorithmic bytes."""
@@ -234,8 +234,8 @@ def kernel_probe(n: int, reps: int):
     nseg = int(load().mbem_i8_segments(n))
     F8 = torch.empty((n, ld8), dtype=torch.uint8, device=dev)
     S8 = torch.empty((n, nseg), dtype=torch.float32, device=dev)
-    ldr = (n + 3) // 4 * 4
-    E = torch.empty((n, ldr), dtype=torch.int32, device=dev)
+    ldr = int(load().mbem_i8e_resid_bytes(n))
+    E = torch.empty((n, ldr), dtype=torch.uint8, device=dev)
     call("mbem_cast_i8e", ptr(A), n, n, n, ptr(F8), ld8, ptr(S8), nseg, ptr(E), ldr, stream())
     for _ in range(reps):
         call("mbem_gemv_i8", ptr(F8), ld8, ptr(S8), nseg, n, n, ptr(xf), ptr(y), 1.0, 0.0,
@@ -601,7 +601,7 @@ def main():
                     help="coarse space of the mixed method's two-level block-Jacobi")
     ap.add_argument("--outer", choices=("fp64", "split"), default="split",
                     help="operator of the mixed method's outer loop with int8 storage: the FP64 "
-                         "matrix, or the int8 codes plus their 32-bit rounding remainders")
+                         "matrix, or the int8 codes plus their 24-bit rounding remainders")
     ap.add_argument("--other-storage-steps", type=int, default=3,
                     help="steps of the mixed method with the other far-field storage (BF16 "
                          "beside int8), reported beside the headline")
@@ -846,9 +846,9 @@ def main():
         roof_main = roof_fp64
     mixed_outer = args.outer if args.storage == "int8" else "fp64"
     outer_desc = ("operator: the FP64 matrix" if mixed_outer == "fp64" else
-                  "operator: int8 codes + 32-bit rounding remainders, FP64 sums")
+                  "operator: int8 codes + 24-bit rounding remainders, FP64 sums")
     roof_outer = (roofline(outer_far[0], outer_far[1], outer_far[2],
-                           "mixed outer matvec, n x n int8 codes + 32-bit remainders, FP64 sums")
+                           "mixed outer matvec, n x n int8 codes + 24-bit remainders, FP64 sums")
                   if main_label == "mixed" and outer_far is not None else None)
     line = {
         "metric": METRIC,

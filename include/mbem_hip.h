@@ -292,14 +292,17 @@ int64_t mbem_i8_segments(int64_t n);
 int mbem_cast_i8(const double *A, int64_t lda, int64_t m, int64_t n, uint8_t *out, int64_t ldo,
                  float *scale, int64_t nseg, hipStream_t stream);
 
-/* mbem_cast_i8 plus the code's rounding remainder: resid[r * ldr + c] =
- * e = rint((A - s q) / s * 2^32) clamped to +-(2^31 - 1), so that
- * s (q + e 2^-32) is A to within s 2^-33 (1e-12 of the segment's largest
- * entry).  resid 16-byte aligned, ldr a multiple of 4.  Same role as
- * mbem_cast_i8 (la/solver.py:178 operator, stored in 5 bytes an entry);
- * its product is mbem_gemv_i8e. */
+/* mbem_cast_i8 plus the code's rounding remainder e = rint((A - s q) / s *
+ * 2^24) clamped to +-(2^23 - 1), so that s (q + e 2^-24) is A to within
+ * s 2^-25 (2e-10 of the segment's largest entry).  The remainders are packed
+ * four to three little-endian 32-bit words (24-bit two's complement): row r's
+ * columns 4k .. 4k+3 are the 12 bytes at resid + r * ldr + 12 k.  resid
+ * 4-byte aligned, ldr (bytes) a multiple of 4 and at least
+ * mbem_i8e_resid_bytes(n).  Same role as mbem_cast_i8 (la/solver.py:178
+ * operator, stored in 4 bytes an entry); its product is mbem_gemv_i8e. */
+int64_t mbem_i8e_resid_bytes(int64_t n);
 int mbem_cast_i8e(const double *A, int64_t lda, int64_t m, int64_t n, uint8_t *out, int64_t ldo,
-                  float *scale, int64_t nseg, int32_t *resid, int64_t ldr, hipStream_t stream);
+                  float *scale, int64_t nseg, uint8_t *resid, int64_t ldr, hipStream_t stream);
 
 /* Hold the near blocks out of A while its int8 copy is built: restore == 0
  * copies them to vals (9 doubles per block, as mbem_near_extract) and zeroes
@@ -315,11 +318,11 @@ int mbem_gemv_i8(const uint8_t *F, int64_t ldf, const float *scale, int64_t nseg
                  hipStream_t stream);
 
 /* y = alpha F x + beta y for the split far field of mbem_cast_i8e,
- * F[r, c] = scale * (out[r, c] - 128 + resid[r, c] 2^-32), FP64 x (16-byte
+ * F[r, c] = scale * (out[r, c] - 128 + e[r, c] 2^-24), FP64 x (16-byte
  * aligned) and FP64 sums: the FGMRES operator of la/solver.py:178 (A v) with
- * the near field added by mbem_bsr_gemv, read from 5 bytes an entry instead
+ * the near field added by mbem_bsr_gemv, read from 4 bytes an entry instead
  * of the FP64 matrix's 8. */
-int mbem_gemv_i8e(const uint8_t *F, int64_t ldf, const int32_t *resid, int64_t ldr,
+int mbem_gemv_i8e(const uint8_t *F, int64_t ldf, const uint8_t *resid, int64_t ldr,
                   const float *scale, int64_t nseg, int64_t m, int64_t n, const double *x,
                   double *y, double alpha, double beta, hipStream_t stream);
 

@@ -75,6 +75,7 @@ _SIGS = {
     "mbem_cast_i8": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p],
     "mbem_near_hold": [_p, _i64, _i64, _p, _p, _i64, _p, _int, _p],
     "mbem_gemv_i8": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
+    "mbem_i8e_resid_bytes": [_i64],
     "mbem_cast_i8e": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _p],
     "mbem_gemv_i8e": [_p, _i64, _p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
     "mbem_bsr_gemv": [_i64, _p, _p, _p, _p, _p, _d, _p],
@@ -153,7 +154,8 @@ _SIGS = {
     "mbem_lr_loss": [_p, _p, _i64, _i64, _p, _p, _p],
 }
 _RESTYPE = {"mbem_nodal_geo_doubles": _i64, "mbem_nodal_table_doubles": _i64,
-            "mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64}
+            "mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64,
+            "mbem_i8e_resid_bytes": _i64}
 
 _lib = None
 

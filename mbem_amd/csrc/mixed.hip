@@ -257,21 +257,25 @@ __global__ __launch_bounds__(256) void k_gemv_lo(const void* __restrict__ F, lon
 // errs by a fixed fraction of each entry, largest on the largest entries).
 constexpr long kSeg = 64;
 
-// With E, the code's rounding remainder is kept too: d = A - s q is exact in
-// FP64 (s q has at most 31 significant bits; for q != 0, A and s q are within
-// a factor 2 of each other), and e = rint(d / s * 2^32), |e| <= 2^31 - 1,
-// so s (q + e 2^-32) is A to within s 2^-33 (and FP64's rounding of d / s):
-// about 1e-12 of the segment's largest entry, from 5 bytes instead of 8.
+// With R3, the code's rounding remainder is kept too: d = A - s q is exact
+// in FP64 (s q has at most 31 significant bits; for q != 0, A and s q are
+// within a factor 2 of each other), and e = rint(d / s * 2^24), |e| <= 2^23
+// - 1, so s (q + e 2^-24) is A to within s 2^-25 (and FP64's rounding of
+// d / s): about 2e-10 of the segment's largest entry, from 4 bytes instead
+// of 8.  The remainders are packed four to three 32-bit words, little-endian
+// 24-bit two's complement: a row's group of columns 4 k .. 4 k + 3 is the 12
+// bytes at 12 k.  (32-bit remainders kept the solution error of the FP64
+// operator, 4.5e-9 at n = 60,000; 24 bits give 5.4e-9, 20 bits 4.6e-8.)
 //
 // one wave per (row, 1024 columns), in 4 passes of 256: lane l encodes 4
 // consecutive columns, so a 64-column segment is one 16-lane row of the wave
 // and its largest magnitude is a reduction inside that row; loads (32 bytes
-// a lane), 4-byte code stores and 16-byte remainder stores are contiguous
+// a lane), 4-byte code stores and 12-byte remainder stores are contiguous
 // across the wave
 __global__ __launch_bounds__(256) void k_cast_i8(const double* __restrict__ A, long lda, long m,
                                                  long n, unsigned char* __restrict__ out, long ldo,
                                                  float* __restrict__ scale, long nseg,
-                                                 int* __restrict__ E, long lde)
+                                                 unsigned char* __restrict__ R3, long ldr)
 {
     const int lane = threadIdx.x & 63;
     const long nblk = (n + 1023) / 1024;
@@ -308,23 +312,26 @@ __global__ __launch_bounds__(256) void k_cast_i8(const double* __restrict__ A, l
             double t = (s > 0.0f) ? rint(v[i] / (double)s) : 0.0;
             t = fmin(fmax(t, -127.0), 127.0);
             q[i] = (unsigned int)((int)t + 128);
-            double f = (s > 0.0f) ? rint((v[i] - (double)s * t) / (double)s * 4294967296.0) : 0.0;
-            f = fmin(fmax(f, -2147483647.0), 2147483647.0);
+            double f = (s > 0.0f) ? rint((v[i] - (double)s * t) / (double)s * 16777216.0) : 0.0;
+            f = fmin(fmax(f, -8388607.0), 8388607.0);
             e[i] = (int)f;
         }
         if (c + 3 < n) {
             *reinterpret_cast<unsigned int*>(o + c) = q[0] | (q[1] << 8) | (q[2] << 16) | (q[3] << 24);
-            if (E) {
-                u32x4 ev = {(unsigned)e[0], (unsigned)e[1], (unsigned)e[2], (unsigned)e[3]};
-                *reinterpret_cast<u32x4*>(E + r * lde + c) = ev;
-            }
         } else {
 #pragma unroll
             for (int i = 0; i < 4; ++i)
-                if (c + i < n) {
-                    o[c + i] = (unsigned char)q[i];
-                    if (E) E[r * lde + c + i] = e[i];
-                }
+                if (c + i < n) o[c + i] = (unsigned char)q[i];
+        }
+        if (R3 && c < n) {
+            // the whole group (entries past n are zero: their v is): the
+            // row pitch covers 3 bytes for every column up to n rounded to 4
+            const unsigned u0 = (unsigned)e[0] & 0xffffffu, u1 = (unsigned)e[1] & 0xffffffu;
+            const unsigned u2 = (unsigned)e[2] & 0xffffffu, u3 = (unsigned)e[3] & 0xffffffu;
+            unsigned* w = reinterpret_cast<unsigned*>(R3 + r * ldr + 3 * c);
+            w[0] = u0 | (u1 << 24);
+            w[1] = (u1 >> 8) | (u2 << 16);
+            w[2] = (u2 >> 16) | (u3 << 8);
         }
         if ((lane & 15) == 0 && c < n) scale[r * nseg + c / kSeg] = s;
     }
@@ -485,25 +492,48 @@ __global__ __launch_bounds__(256) void k_gemv_i8(const unsigned char* __restrict
     }
 }
 
-// sum over a group of four columns of (u - 128 + e 2^-32) x in FP64, given
+// a group's three remainder words (four 24-bit two's-complement values,
+// little-endian) from a 4-byte aligned address
+struct Rem3 {
+    unsigned w0, w1, w2;
+};
+
+__device__ __forceinline__ Rem3 load_rem3(const unsigned char* p)
+{
+    const unsigned* q = reinterpret_cast<const unsigned*>(p);
+    return Rem3{__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1),
+                __builtin_nontemporal_load(q + 2)};
+}
+
+// the i-th 24-bit value of a group, sign-extended
+__device__ __forceinline__ int rem24(const Rem3& e, int i)
+{
+    const unsigned v = (i == 0) ? e.w0
+                       : (i == 1) ? __builtin_amdgcn_alignbit(e.w1, e.w0, 24)
+                       : (i == 2) ? __builtin_amdgcn_alignbit(e.w2, e.w1, 16)
+                                  : (e.w2 >> 8);
+    return ((int)(v << 8)) >> 8;
+}
+
+// sum over a group of four columns of (u - 128 + e 2^-24) x in FP64, given
 // sx = the group's sum of x: the codes' offset is taken out once per group
-__device__ __forceinline__ double group_i8e(const unsigned w, const u32x4 e, const dbl2 xa,
+__device__ __forceinline__ double group_i8e(const unsigned w, const Rem3& e, const dbl2 xa,
                                             const dbl2 xb, const double sx)
 {
     const double ux = (double)(w & 0xffu) * xa.x + (double)((w >> 8) & 0xffu) * xa.y +
                       (double)((w >> 16) & 0xffu) * xb.x + (double)(w >> 24) * xb.y;
-    const double ex = (double)(int)e.x * xa.x + (double)(int)e.y * xa.y + (double)(int)e.z * xb.x +
-                      (double)(int)e.w * xb.y;
-    return (ux - 128.0 * sx) + ex * 0x1p-32;
+    const double ex = (double)rem24(e, 0) * xa.x + (double)rem24(e, 1) * xa.y +
+                      (double)rem24(e, 2) * xb.x + (double)rem24(e, 3) * xb.y;
+    return (ux - 128.0 * sx) + ex * 0x1p-24;
 }
 
 // y = alpha (F x) + beta y in FP64 for the split far field
-// F[r, c] = s (u - 128 + e 2^-32) (codes u, remainders e, the per-segment
-// scales s of k_cast_i8), FP64 x: the outer operator of the mixed method.
-// One wave per R rows.  In each 1024-column step lane l takes four groups of
-// four columns, c = c0 + 256 g + 4 l (g < 4): per group and row one 4-byte
-// load of codes and one 16-byte load of remainders, contiguous across the
-// wave (256 and 1,024 bytes).  A group lies in segment c0 / 64 + 4 g + l / 16;
+// F[r, c] = s (u - 128 + e 2^-24) (codes u, 24-bit remainders e, the
+// per-segment scales s of k_cast_i8), FP64 x: the outer operator of the
+// mixed method.  One wave per R rows.  In each 1024-column step lane l takes
+// four groups of four columns, c = c0 + 256 g + 4 l (g < 4): per group and
+// row one 4-byte load of codes and three 4-byte loads of remainders,
+// contiguous across the wave (256 and 768 bytes).  A group lies in segment c0 / 64 + 4 g + l / 16;
 // a row's 16 scales of the step are one load (lane l holds segment
 // c0 / 64 + l % 16) and each group takes its own by a lane permute.  x goes
 // through LDS 2048 columns at a time, double-buffered as in k_gemv_i8 (the
@@ -511,7 +541,7 @@ __device__ __forceinline__ double group_i8e(const unsigned w, const u32x4 e, con
 // are clamped and not stored).  Requires 256 threads per block.
 template <int R>
 __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restrict__ F, long ldf,
-                                                  const int* __restrict__ E, long lde,
+                                                  const unsigned char* __restrict__ R3, long ldr,
                                                   const float* __restrict__ S, long nseg, long m,
                                                   long n, const double* __restrict__ x,
                                                   double* __restrict__ y, double alpha, double beta)
@@ -522,7 +552,7 @@ __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restric
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const long r0 = ((long)blockIdx.x * (blockDim.x >> 6) + wv) * R;
     const unsigned char* fb[R];
-    const int* eb[R];
+    const unsigned char* eb[R];
     const float* sb[R];
     double acc[R];
 #pragma unroll
@@ -530,7 +560,7 @@ __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restric
         acc[r] = 0.0;
         const long rr = (r0 + r < m) ? r0 + r : m - 1;
         fb[r] = F + rr * ldf;
-        eb[r] = E + rr * lde;
+        eb[r] = R3 + rr * ldr;
         sb[r] = S + rr * nseg;
     }
     // columns within a row fit 32 bits (the entry point checks n < 2^31)
@@ -558,7 +588,7 @@ __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restric
         for (int st = 0; st < 2; ++st) {
             const unsigned c0 = base + 1024 * st;
             unsigned w[4][R];
-            u32x4 ev[4][R];
+            Rem3 ev[4][R];
             float sv[R];
 #pragma unroll
             for (int r = 0; r < R; ++r) {
@@ -567,7 +597,7 @@ __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restric
                 for (int g = 0; g < 4; ++g) {
                     const unsigned c = c0 + 256 * g + 4 * lane;
                     w[g][r] = __builtin_nontemporal_load(reinterpret_cast<const unsigned*>(fb[r] + c));
-                    ev[g][r] = __builtin_nontemporal_load(reinterpret_cast<const u32x4*>(eb[r] + c));
+                    ev[g][r] = load_rem3(eb[r] + 3 * c);
                 }
             }
 #pragma unroll
@@ -598,14 +628,16 @@ __global__ __launch_bounds__(256) void k_gemv_i8e(const unsigned char* __restric
         for (int r = 0; r < R; ++r)
             acc[r] += (double)sb[r][c / 64] *
                       group_i8e(*reinterpret_cast<const unsigned*>(fb[r] + c),
-                                *reinterpret_cast<const u32x4*>(eb[r] + c), xa, xb, sx);
+                                load_rem3(eb[r] + 3 * c), xa, xb, sx);
     }
     for (unsigned c = 4 * ng + lane; c < (unsigned)n; c += 64) {
         const double xc = x[c];
+        const unsigned g4 = c & ~3u, i = c & 3u;
 #pragma unroll
         for (int r = 0; r < R; ++r)
             acc[r] += (double)sb[r][c / 64] *
-                      ((((double)fb[r][c] - 128.0) + (double)eb[r][c] * 0x1p-32) * xc);
+                      ((((double)fb[r][c] - 128.0) +
+                        (double)rem24(load_rem3(eb[r] + 3 * g4), (int)i) * 0x1p-24) * xc);
     }
 #pragma unroll
     for (int r = 0; r < R; ++r) {
@@ -898,21 +930,24 @@ int mbem_cast_i8(const double* A, int64_t lda, int64_t m, int64_t n, uint8_t* ou
     const long blocks = (waves + 3) / 4;
     if (blocks > 0x7fffffffL) return MBEM_EINVAL;
     hipLaunchKernelGGL(k_cast_i8, dim3((unsigned)blocks), dim3(256), 0, stream, A, lda, m, n, out, ldo, scale, nseg,
-                       (int*)nullptr, 0L);
+                       (unsigned char*)nullptr, 0L);
     return launch_status();
 }
 
+int64_t mbem_i8e_resid_bytes(int64_t n) { return n > 0 ? 3 * ((n + 3) / 4 * 4) : 0; }
+
 int mbem_cast_i8e(const double* A, int64_t lda, int64_t m, int64_t n, uint8_t* out, int64_t ldo,
-                  float* scale, int64_t nseg, int32_t* resid, int64_t ldr, hipStream_t stream)
+                  float* scale, int64_t nseg, uint8_t* resid, int64_t ldr, hipStream_t stream)
 {
     if (m < 0 || n < 0 ||
-        (m > 0 && n > 0 && (!A || !out || !scale || !resid || lda < n || ldo < n || ldr < n)))
+        (m > 0 && n > 0 && (!A || !out || !scale || !resid || lda < n || ldo < n ||
+                            ldr < mbem_i8e_resid_bytes(n))))
         return MBEM_EINVAL;
     if (nseg != mbem_i8_segments(n)) return MBEM_EINVAL;
     if (m == 0 || n == 0) return MBEM_OK;
-    // 16-byte stores of the packed codes and of the remainders
+    // 16-byte stores of the packed codes, 4-byte stores of the remainders
     if ((reinterpret_cast<uintptr_t>(out) & 15) || (ldo % 16) ||
-        (reinterpret_cast<uintptr_t>(resid) & 15) || (ldr % 4))
+        (reinterpret_cast<uintptr_t>(resid) & 3) || (ldr % 4))
         return MBEM_EINVAL;
     const long waves = m * ((n + 1023) / 1024);
     const long blocks = (waves + 3) / 4;
@@ -939,21 +974,22 @@ int mbem_gemv_i8(const uint8_t* F, int64_t ldf, const float* scale, int64_t nseg
     return launch_status();
 }
 
-int mbem_gemv_i8e(const uint8_t* F, int64_t ldf, const int32_t* resid, int64_t ldr,
+int mbem_gemv_i8e(const uint8_t* F, int64_t ldf, const uint8_t* resid, int64_t ldr,
                   const float* scale, int64_t nseg, int64_t m, int64_t n, const double* x,
                   double* y, double alpha, double beta, hipStream_t stream)
 {
-    if (m < 0 || n < 0 || (m > 0 && (!F || !resid || !y || !scale || ldf < n || ldr < n)) ||
+    if (m < 0 || n < 0 ||
+        (m > 0 && (!F || !resid || !y || !scale || ldf < n || ldr < mbem_i8e_resid_bytes(n))) ||
         (n > 0 && !x))
         return MBEM_EINVAL;
     if (nseg < mbem_i8_segments(n)) return MBEM_EINVAL;
     if (m == 0) return MBEM_OK;
-    // 4-byte code loads and 16-byte remainder and x loads at multiples of 4 columns
+    // 4-byte code and remainder loads, 16-byte x loads, at multiples of 4 columns
     if ((reinterpret_cast<uintptr_t>(F) & 15) || (ldf % 16) ||
-        (reinterpret_cast<uintptr_t>(resid) & 15) || (ldr % 4) ||
+        (reinterpret_cast<uintptr_t>(resid) & 3) || (ldr % 4) ||
         (reinterpret_cast<uintptr_t>(x) & 15))
         return MBEM_EINVAL;
-    if (n > INT32_MAX) return MBEM_EINVAL;     // 32-bit column offsets
+    if (n > INT32_MAX / 3) return MBEM_EINVAL;     // 32-bit remainder byte offsets (3 per column)
     constexpr int R = MBEM_I8E_ROWS_PER_WAVE;
     const long waves = (m + R - 1) / R;
     const long blocks = (waves + 3) / 4;

@@ -12,13 +12,14 @@ Operator split A = F + N on the device:
 Solve: the outer loop is FP64 FGMRES on the FP64 operator (Arnoldi, Givens
 and the true residual all FP64), so it is iterative refinement -- either the
 assembled FP64 matrix or, with int8 storage, ``SplitOuter``: the int8 codes
-plus their 32-bit rounding remainders, every far entry within 1e-12 of its
-segment's largest magnitude, from 5 bytes instead of 8; its
+plus their 24-bit rounding remainders, every far entry within 2e-10 of its
+segment's largest magnitude, from 4 bytes instead of 8; its
 right preconditioner is an inner FGMRES on the mixed operator F + N,
 preconditioned by block Gauss-Seidel / block-Jacobi with FP32 chunk
 inverses and mixed-operator couplings, run to a loose tolerance.  The error
 against the FP64 solution is set by the outer rtol (the split operator's
-entry errors, about 1e-12 of each segment's largest, stay far below it).
+entry errors, about 2e-10 of each segment's largest, add 0.8e-9 to 4.5e-9
+at n = 60,000).
 """
 from __future__ import annotations
 
@@ -129,10 +130,10 @@ class MixedOperator:
             self.nseg = int(_lib.load().mbem_i8_segments(n))
             self.F = torch.empty((n, self.ldf), dtype=torch.uint8, device=dev)
             self.S = torch.empty((n, max(self.nseg, 1)), dtype=torch.float32, device=dev)
-            # with split, each code's rounding remainder too (32 bits): the
-            # outer operator of SplitOuter
-            self.lde = ((n + 3) // 4) * 4
-            self.E = torch.empty((n, self.lde), dtype=torch.int32, device=dev) if split else None
+            # with split, each code's rounding remainder too (24 bits, packed
+            # four to three words): the outer operator of SplitOuter
+            self.lde = int(_lib.load().mbem_i8e_resid_bytes(n))
+            self.E = torch.empty((n, self.lde), dtype=torch.uint8, device=dev) if split else None
             hold = (self.nbr, ptr(self.rowptr), ptr(self.col), self.nnzb, ptr(self.vals))
             call("mbem_near_hold", ptr(A), A.stride(0), *hold, 0, stream())
             try:
@@ -214,11 +215,11 @@ class MixedOperator:
 
 class SplitOuter:
     """The outer loop's operator from the split far field: A v as
-    s (u - 128 + e 2^-32) v over the far entries (int8 codes u and 32-bit
+    s (u - 128 + e 2^-24) v over the far entries (int8 codes u and 24-bit
     remainders e, FP64 v and FP64 sums) plus the FP64 near blocks.  Every far
-    entry is A's to within s 2^-33 (s = its segment's largest magnitude /
-    127, so about 1e-12 of that magnitude; the near blocks are exact), read
-    from 5 bytes instead of 8: the FP64 pass over A in each outer step of the
+    entry is A's to within s 2^-25 (s = its segment's largest magnitude /
+    127, so about 2e-10 of that magnitude; the near blocks are exact), read
+    from 4 bytes instead of 8: the FP64 pass over A in each outer step of the
     solve (la/solver.py:178) becomes a pass over 5/8 of the bytes."""
 
     def __init__(self, op: MixedOperator):
@@ -226,7 +227,7 @@ class SplitOuter:
             raise ValueError("SplitOuter needs a MixedOperator built with split=True")
         self.op = op
         self.n = op.n
-        self.bytes = op.n * op.ldf + op.n * op.lde * 4 + op.n * op.nseg * 4 + op.near_bytes
+        self.bytes = op.n * op.ldf + op.n * op.lde + op.n * op.nseg * 4 + op.near_bytes
 
     def apply_into(self, v, out):
         o = self.op
@@ -407,7 +408,7 @@ class MixedReport:
         c = "" if not self.coarse else (f", coarse {self.coarse['basis']} rank "
                                         f"{self.coarse['rank']}")
         o = ("FP64 outer" if self.outer_operator == "fp64" else
-             "outer on int8 codes + 32-bit remainders, FP64 sums")
+             "outer on int8 codes + 24-bit remainders, FP64 sums")
         return (f"mixed ({self.storage} far field, FP64 near, {o}{c}): outer {self.outer}, "
                 f"inner iterations {self.inner_iterations}")
 
@@ -436,9 +437,9 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     solves (``_RecycledInner``); ``coarse`` ("const", "rigid", "linear")
     adds the coarse correction of la/coarse.py to the inner block-Jacobi.
     ``outer`` is the outer loop's operator: "fp64" the assembled FP64 matrix,
-    "split" (int8 storage) ``SplitOuter``, the int8 codes plus their 32-bit
-    rounding remainders (every far entry within 1e-12 of its segment's
-    largest, FP64 sums, near blocks exact), 5/8 of the FP64 bytes per outer
+    "split" (int8 storage) ``SplitOuter``, the int8 codes plus their 24-bit
+    rounding remainders (every far entry within 2e-10 of its segment's
+    largest, FP64 sums, near blocks exact), half the FP64 bytes per outer
     step."""
     if outer not in ("fp64", "split"):
         raise ValueError(f"outer {outer!r} (expected 'fp64' or 'split')")

@@ -11,10 +11,11 @@ does, so the GPU bytes, scales and products can be checked bit for bit:
   is s (byte - 128).
 
 The split form (mbem_cast_i8e, the mixed method's outer operator) keeps each
-code's rounding remainder as well: e = rint((A - s q) / s * 2^32) (the
-difference, the division and the product by 2^32 in float64), clamped to
-+-(2^31 - 1), e = 0 where s = 0; the represented value is
-s (byte - 128 + e 2^-32).
+code's rounding remainder as well: e = rint((A - s q) / s * 2^24) (the
+difference, the division and the product by 2^24 in float64), clamped to
++-(2^23 - 1), e = 0 where s = 0; the represented value is
+s (byte - 128 + e 2^-24).  On the device the remainders are packed four to
+three little-endian 32-bit words of 24-bit two's complement (``pack24``).
 """
 from __future__ import annotations
 
@@ -60,8 +61,8 @@ def cast_split(A) -> tuple:
     sd = scales[:, np.arange(n) // SEG].astype(np.float64)
     q = codes.astype(np.float64) - 128.0
     with np.errstate(divide="ignore", invalid="ignore"):
-        e = np.where(sd > 0, np.rint((A - sd * q) / np.where(sd > 0, sd, 1.0) * 4294967296.0), 0.0)
-    e = np.clip(e, -2147483647.0, 2147483647.0).astype(np.int32)
+        e = np.where(sd > 0, np.rint((A - sd * q) / np.where(sd > 0, sd, 1.0) * 16777216.0), 0.0)
+    e = np.clip(e, -8388607.0, 8388607.0).astype(np.int32)
     return codes, scales, e
 
 
@@ -69,5 +70,18 @@ def dequantize_split(codes, scales, resid) -> np.ndarray:
     """float64 matrix the split storage represents."""
     n = codes.shape[1]
     seg = np.arange(n) // SEG
-    v = (codes.astype(np.float64) - 128.0) + resid.astype(np.float64) * 2.0 ** -32
+    v = (codes.astype(np.float64) - 128.0) + resid.astype(np.float64) * 2.0 ** -24
     return v * scales[:, seg].astype(np.float64)
+
+
+def pack24(resid) -> np.ndarray:
+    """(m, 3 ceil(n/4) 4) bytes: each row's remainders, zero-padded to a
+    multiple of 4 columns, as 24-bit little-endian two's complement."""
+    resid = np.asarray(resid, dtype=np.int64)
+    m, n = resid.shape
+    n4 = -(-n // 4) * 4
+    r = np.zeros((m, n4), dtype=np.int64)
+    r[:, :n] = resid
+    u = (r & 0xFFFFFF).astype(np.uint32)
+    b = np.stack([u & 0xFF, (u >> 8) & 0xFF, (u >> 16) & 0xFF], axis=-1).astype(np.uint8)
+    return b.reshape(m, 3 * n4)

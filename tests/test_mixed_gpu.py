@@ -11,10 +11,10 @@ the 2,048-triangle box with its dipping fault.
     FP32 vector to FP32 accumulation (1e-5 of the sum of |terms|), and
     holding the near blocks out of A and writing them back leaves A
     bit-identical;
-  * the split form (codes plus 32-bit rounding remainders) equals the
-    oracle's bit for bit and its FP64 product the oracle's split matrix
-    times x to FP64 rounding (1e-14 of the sum of |terms|), within
-    s 2^-33 per entry of A itself;
+  * the split form (codes plus 24-bit rounding remainders, packed four to
+    three words) equals the oracle's bit for bit and its FP64 product the
+    oracle's split matrix times x to FP64 rounding (1e-14 of the sum of
+    |terms|), within s 2^-25 per entry of A itself;
   * the nested FGMRES (FP64 outer on the exact operator, or on the split
     operator) reaches the FP64 true residual 1e-12 and the oracle's LU
     solution to 1e-9, with and without the recycled space across inner
@@ -69,9 +69,14 @@ def test_mixed_matvec(c1, storage, tol):
 def test_mixed_solve(c1, storage, recycle, outer):
     """The last case is the bench's method: int8 far field, 90 recycled
     vectors, rigid coarse space and the outer loop on the split operator
-    (int8 codes + 32-bit remainders); its residual is also taken with the
-    oracle's FP64 matrix."""
-    from mbem_amd.la.mixed import solve_mixed
+    (int8 codes + 24-bit remainders).  The outer loop then solves the split
+    operator's system, not A's: its solution equals the oracle's LU solution
+    of the oracle's split operator (far field through oracle/int8.cast_split,
+    near blocks exact) to 1e-9, and differs from A's LU solution by the
+    representation's floor (entries within s 2^-25 of A's: 2.9e-9 here),
+    below the 1e-8 the method is for."""
+    from mbem_amd.la.mixed import near_structure, solve_mixed
+    from oracle import int8 as oi
     asm = c1["asm"]
     x, rep = solve_mixed(asm, rtol=1e-12, storage=storage, recycle=recycle, outer=outer,
                          coarse="rigid" if outer == "split" else None)
@@ -81,8 +86,22 @@ def test_mixed_solve(c1, storage, recycle, outer):
     print(f"{rep}; rel err vs oracle LU {err:.2e}, oracle FP64 residual {res64:.2e}")
     assert rep.converged and rep.outer.true_relres < 1e-12
     assert rep.outer_operator == outer
-    assert err < 1e-9
-    assert res64 < 2e-12
+    if outer == "fp64":
+        assert err < 1e-9 and res64 < 2e-12
+        return
+    A = c1["A"]
+    rowptr, col = near_structure(asm, 2.0)
+    near = np.zeros(A.shape, dtype=bool)
+    for i in range(rowptr.size - 1):
+        for j in col[rowptr[i]:rowptr[i + 1]]:
+            near[3 * i:3 * i + 3, 3 * j:3 * j + 3] = True
+    codes, scales, resid = oi.cast_split(np.where(near, 0.0, A))
+    A_split = np.where(near, A, oi.dequantize_split(codes, scales, resid))
+    x_split = np.linalg.solve(A_split, c1["b"])
+    err_split = np.linalg.norm(xh - x_split) / np.linalg.norm(x_split)
+    print(f"split: rel err vs the oracle split operator's LU {err_split:.2e}")
+    assert err_split < 1e-9
+    assert err < 1e-8
 
 
 @pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030), (45, 5000),
@@ -133,7 +152,7 @@ def test_int8_cast_and_product_match_oracle(m, n):
 @pytest.mark.parametrize("m,n", [(37, 2500), (5, 16), (9, 1040), (6, 1030), (45, 5000),
                                  (3, 8208), (13, 4099), (7, 2050)])
 def test_int8_split_cast_and_product_match_oracle(m, n):
-    """The split form (codes, scales and 32-bit remainders) bit for bit
+    """The split form (codes, scales and packed 24-bit remainders) bit for bit
     against the oracle, and its FP64 product against the oracle's split
     matrix times x: 2500 columns leave one LDS tile (2048) and tail groups of
     four, 5000 two tiles with waves past the last row, 8208 four tiles and
@@ -152,15 +171,16 @@ def test_int8_split_cast_and_product_match_oracle(m, n):
     Ad[:, :n] = torch.as_tensor(A, device=dev)
     nseg = int(load().mbem_i8_segments(n))
     ldf = ((n + 15) // 16) * 16
-    ldr = ((n + 3) // 4) * 4 + 4
+    nb = int(load().mbem_i8e_resid_bytes(n))
+    ldr = nb + 8
     F = torch.zeros((m, ldf), dtype=torch.uint8, device=dev)
     S = torch.zeros((m, nseg), dtype=torch.float32, device=dev)
-    E = torch.zeros((m, ldr), dtype=torch.int32, device=dev)
+    E = torch.zeros((m, ldr), dtype=torch.uint8, device=dev)
     call("mbem_cast_i8e", ptr(Ad), lda, m, n, ptr(F), ldf, ptr(S), nseg, ptr(E), ldr, stream())
     codes, scales, resid = oi.cast_split(A)
     assert np.array_equal(F[:, :n].cpu().numpy(), codes)
     assert np.array_equal(S.cpu().numpy(), scales)
-    assert np.array_equal(E[:, :n].cpu().numpy(), resid)
+    assert np.array_equal(E[:, :nb].cpu().numpy(), oi.pack24(resid))
     x = rng.standard_normal(n)
     xd = torch.as_tensor(x, device=dev)
     y0 = rng.standard_normal(m)
@@ -177,8 +197,8 @@ def test_int8_split_cast_and_product_match_oracle(m, n):
     err = np.abs(yh - ref) / scale
     print(f"split {m}x{n}: product err / |terms| max {err.max():.2e}")
     assert err.max() < 1e-14
-    # and against A itself: the split form is A to within s 2^-33 per entry
-    bound = 0.5 * ((step * 2.0 ** -33 + np.abs(A) * 2.0 ** -52) @ np.abs(x)) + 1e-14 * scale
+    # and against A itself: the split form is A to within s 2^-25 per entry
+    bound = 0.5 * ((step * 2.0 ** -25 + np.abs(A) * 2.0 ** -52) @ np.abs(x)) + 1e-14 * scale
     assert np.all(np.abs(yh - (-0.5 * (A @ x) + 2.0 * y0)) <= bound * 1.01)
 
 
