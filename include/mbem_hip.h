@@ -631,6 +631,14 @@ int mbem_coarse_factor(double *G, int64_t K, double delta, double *d, int32_t *i
 int64_t mbem_gram_work(int64_t k, int64_t n);
 int mbem_gram(const double *X, int64_t ldx, int64_t k, int64_t n, double *G, double *work,
               hipStream_t stream);
+/* The coarse projection with Q^T (K x n, row-major, pitch ldq) stored in
+ * FP32, FP64 vectors and sums: s = Q^T r, rest = r - Q s.  work:
+ * mbem_coarse_project_work(K, n) doubles.  The deflation step of the
+ * two-level block-Jacobi (no counterpart in the reference's preconditioner,
+ * la/preconditioner.py:421-470, which it wraps). */
+int64_t mbem_coarse_project_work(int64_t K, int64_t n);
+int mbem_coarse_project_f32(const float *Qt, int64_t ldq, int64_t K, int64_t n, const double *r,
+                            double *s, double *rest, double *work, hipStream_t stream);
 int mbem_coarse_expand(const double *u, const int64_t *perm, const double *Wp, const int64_t *grp,
                        int64_t nt, int q, double alpha, double *x, hipStream_t stream);
 

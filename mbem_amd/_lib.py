@@ -130,6 +130,8 @@ _SIGS = {
     "mbem_coarse_image": [_p, _i64, _i64, _p, _p, _p, _i64, _int, _p, _i64, _p],
     "mbem_coarse_expand": [_p, _p, _p, _p, _i64, _int, _d, _p, _p],
     "mbem_gram_work": [_i64, _i64],
+    "mbem_coarse_project_work": [_i64, _i64],
+    "mbem_coarse_project_f32": [_p, _i64, _i64, _i64, _p, _p, _p, _p, _p],
     "mbem_coarse_image_i8": [_p, _i64, _p, _i64, _i64, _p, _p, _p, _i64, _int, _p, _i64, _p],
     "mbem_coarse_image_bsr": [_i64, _p, _p, _p, _p, _p, _p, _int, _p, _i64, _p],
     "mbem_gram": [_p, _i64, _i64, _i64, _p, _p, _p],
@@ -155,7 +157,7 @@ _SIGS = {
 }
 _RESTYPE = {"mbem_nodal_geo_doubles": _i64, "mbem_nodal_table_doubles": _i64,
             "mbem_multidot_work": _i64, "mbem_gram_work": _i64, "mbem_i8_segments": _i64, "mbem_colsum_work": _i64, "mbem_lu_inv_doubles": _i64,
-            "mbem_i8e_resid_bytes": _i64}
+            "mbem_i8e_resid_bytes": _i64, "mbem_coarse_project_work": _i64}
 
 _lib = None
 

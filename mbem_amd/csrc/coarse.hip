@@ -323,6 +323,74 @@ int expand_q(const double* u, const int64_t* perm, const double* Wp, const int64
     return launch_status();
 }
 
+// The coarse projection with Q^T stored in FP32 (FP64 vectors and sums):
+// s = Q^T r and rest = r - Q s.  Q^T is K x n row-major (a row per coarse
+// direction).  part[c K + row] = sum over the 512 columns of chunk c of
+// Q^T[row, j] r[j]: a wave per (row, chunk), 16-byte loads of four floats
+// and two 16-byte loads of r when rows and r are 16-byte aligned.
+constexpr long kQChunk = 512;
+
+__global__ __launch_bounds__(256) void k_qdot_f32(const float* __restrict__ Q, long ldq, long K,
+                                                  long n, const double* __restrict__ r,
+                                                  double* __restrict__ part, int vec)
+{
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    const long c = blockIdx.y;
+    if (row >= K) return;
+    const long j0 = c * kQChunk;
+    const float* q = Q + row * ldq;
+    double a = 0.0;
+    if (vec && j0 + kQChunk <= n) {
+#pragma unroll
+        for (int t = 0; t < (int)(kQChunk / 256); ++t) {
+            const long j = j0 + t * 256 + 4 * lane;
+            const float4 qv = *reinterpret_cast<const float4*>(q + j);
+            const double2 r0 = *reinterpret_cast<const double2*>(r + j);
+            const double2 r1 = *reinterpret_cast<const double2*>(r + j + 2);
+            a += ((double)qv.x * r0.x + (double)qv.y * r0.y) + ((double)qv.z * r1.x + (double)qv.w * r1.y);
+        }
+    } else {
+        const long j1 = (j0 + kQChunk < n) ? j0 + kQChunk : n;
+        for (long j = j0 + lane; j < j1; j += 64) a += (double)q[j] * r[j];
+    }
+    a = wave_sum(a);
+    if (lane == 0) part[c * K + row] = a;
+}
+
+// s[row] = sum_c part[c K + row]: a wave per row, lanes over the parts
+__global__ __launch_bounds__(256) void k_qreduce(const double* __restrict__ part, long K, long nparts,
+                                                 double* __restrict__ s)
+{
+    const int lane = threadIdx.x & 63;
+    const long row = (long)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    if (row >= K) return;
+    double a = 0.0;
+    for (long c = lane; c < nparts; c += 64) a += part[c * K + row];
+    a = wave_sum(a);
+    if (lane == 0) s[row] = a;
+}
+
+// rest[j] = r[j] - sum_row Q^T[row, j] s[row]: a workgroup per 64 columns,
+// its four waves taking every fourth row, combined in LDS in a fixed order
+__global__ __launch_bounds__(256) void k_qaxpy_f32(const float* __restrict__ Q, long ldq, long K,
+                                                   long n, const double* __restrict__ s,
+                                                   const double* __restrict__ r,
+                                                   double* __restrict__ rest)
+{
+    __shared__ double red[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const long j = (long)blockIdx.x * 64 + lane;
+    double a = 0.0;
+    if (j < n) {
+#pragma unroll 4
+        for (long row = wv; row < K; row += 4) a += (double)Q[row * ldq + j] * s[row];
+    }
+    red[wv][lane] = a;
+    __syncthreads();
+    if (wv == 0 && j < n) rest[j] = r[j] - ((red[0][lane] + red[1][lane]) + (red[2][lane] + red[3][lane]));
+}
+
 }  // namespace
 
 int mbem_coarse_image(const double* A, int64_t lda, int64_t m, const int64_t* perm,
@@ -367,6 +435,32 @@ int mbem_coarse_image_bsr(int64_t nbr, const int64_t* rowptr, const int64_t* col
     case 6: return image_bsr_q<6>(nbr, rowptr, col, vals, inv, grp, Wp, Ct, ldc, stream);
     default: return image_bsr_q<12>(nbr, rowptr, col, vals, inv, grp, Wp, Ct, ldc, stream);
     }
+}
+
+int64_t mbem_coarse_project_work(int64_t K, int64_t n)
+{
+    int64_t parts = (n + kQChunk - 1) / kQChunk;
+    if (parts < 1) parts = 1;
+    return parts * (K > 0 ? K : 1);
+}
+
+int mbem_coarse_project_f32(const float* Qt, int64_t ldq, int64_t K, int64_t n, const double* r,
+                            double* s, double* rest, double* work, hipStream_t stream)
+{
+    if (K < 0 || n < 0 || (K > 0 && n > 0 && (!Qt || !r || !s || !rest || !work || ldq < n)))
+        return MBEM_EINVAL;
+    if (K == 0 || n == 0) return MBEM_OK;
+    const int64_t parts = (n + kQChunk - 1) / kQChunk;
+    if (parts > 65535 || (K + 3) / 4 > 0x7fffffffL) return MBEM_EINVAL;
+    const int vec = (ldq % 4 == 0) && (reinterpret_cast<uintptr_t>(Qt) % 16 == 0) &&
+                    (reinterpret_cast<uintptr_t>(r) % 16 == 0);
+    hipLaunchKernelGGL(k_qdot_f32, dim3((unsigned)((K + 3) / 4), (unsigned)parts), dim3(256), 0, stream,
+                       Qt, (long)ldq, (long)K, (long)n, r, work, vec);
+    hipLaunchKernelGGL(k_qreduce, dim3((unsigned)((K + 3) / 4)), dim3(256), 0, stream, work, (long)K,
+                       (long)parts, s);
+    hipLaunchKernelGGL(k_qaxpy_f32, dim3((unsigned)((n + 63) / 64)), dim3(256), 0, stream, Qt, (long)ldq,
+                       (long)K, (long)n, s, r, rest);
+    return launch_status();
 }
 
 int mbem_coarse_expand(const double* u, const int64_t* perm, const double* Wp, const int64_t* grp,

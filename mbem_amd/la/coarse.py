@@ -144,6 +144,8 @@ class CoarseSpace:
         self.m = 0 if self.extra is None else self.extra.shape[1]
         self.K_local = self.G * self.q
         K = self.K_local + self.m
+        # the image and projection of an int8 operator run in FP32 (below)
+        self.q32 = op is not None and getattr(op, "i8", False) and self.K_local > 0
         self.K = K
         self.perm = _lib.upload(perm, I64)
         self.Wp = _lib.upload(Wp, F64)
@@ -162,14 +164,23 @@ class CoarseSpace:
             del AE
         if self.K_local == 0:
             self.source = "FP64 matrix (global columns only)"
-        elif op is not None and getattr(op, "i8", False):
+        elif self.q32:
             # the image of the mixed operator F + N itself: the int8 far
             # field (n^2 bytes) and the FP64 near blocks
             inv = np.empty(n, np.int64)
             inv[perm] = np.arange(n, dtype=np.int64)
             self._inv = _lib.upload(inv, I64)
-            call("mbem_coarse_image_i8", ptr(op.F), op.ldf, ptr(op.S), op.nseg, n, ptr(self.perm),
-                 ptr(self.Wp), ptr(self._gptr), self.G, self.q, ptr(self._Ct), n, stream())
+            # row chunk by row chunk behind the operator's cast events (when
+            # this runs on another stream than the cast, the image of the
+            # first chunks overlaps the cast of the later ones)
+            chunks = getattr(op, "cast_events", None) or [(0, n, None)]
+            cur = torch.cuda.current_stream()
+            for r0, r1, ev in chunks:
+                if ev is not None:
+                    cur.wait_event(ev)
+                call("mbem_coarse_image_i8", ptr(op.F[r0:r1]), op.ldf, ptr(op.S[r0:r1]), op.nseg,
+                     r1 - r0, ptr(self.perm), ptr(self.Wp), ptr(self._gptr), self.G, self.q,
+                     ptr(self._Ct[:, r0:]), n, stream())
             call("mbem_coarse_image_bsr", op.nbr, ptr(op.rowptr), ptr(op.col), ptr(op.vals),
                  ptr(self._inv), ptr(self.grp), ptr(self.Wp), self.q, ptr(self._Ct), n, stream())
             self.source = "int8 far field + FP64 near field"
@@ -188,10 +199,14 @@ class CoarseSpace:
         """Queue the factorisation behind the image (no host wait): the
         Gram matrix scaled to unit diagonal (displacement and traction
         columns differ by ~1e12) plus ``drop`` I, its Cholesky factor and
-        inverse, T, and Q^T = T^T Ct with one GEMM.  The regularisation
-        keeps every direction of the scaled Gram above ``drop`` and fades
-        out the (near-)dependent ones, so no pivoting or rank decision
-        needs the host."""
+        inverse, T, and Q^T = T^T Ct with one GEMM.  Built from an int8
+        operator (``op``), Q^T is kept in FP32 for the projections (FP64
+        vectors and sums): that operator is A to 1.8e-4, FP32 changes
+        nothing it does, and it halves the 2 K n bytes a projection reads;
+        otherwise Q^T stays FP64.  The regularisation keeps every direction
+        of the scaled Gram above ``drop`` and fades out the
+        (near-)dependent ones, so no pivoting or rank decision needs the
+        host."""
         if self.Qt is not None:
             return self
         K, n, dev = self.K, self.n, self._Ct.device
@@ -202,20 +217,33 @@ class CoarseSpace:
         call("mbem_coarse_factor", ptr(self._gram), K, float(self.drop), ptr(d), ptr(info),
              ptr(Tt), ptr(self.T), stream())
         self.info = info
-        self.Qt = torch.empty((K, n), dtype=F64, device=dev)
+        # Q^T in FP64 (row pitch ldq, 16-byte rows); for the FP32 projection
+        # its FP32 copy
+        self.ldq = ((n + 3) // 4) * 4 if self.q32 else n
+        q64 = torch.empty((K, self.ldq), dtype=F64, device=dev)
         call("mbem_dgemm_rowmajor", K, n, K, 1.0, ptr(Tt), K, ptr(self._Ct), n, 0.0,
-             ptr(self.Qt), n, stream())
+             ptr(q64), self.ldq, stream())
+        if self.q32:
+            self.Qt = torch.empty((K, self.ldq), dtype=torch.float32, device=dev)
+            call("mbem_vec_to_f32", ptr(q64), K * self.ldq, ptr(self.Qt), stream())
+            q64.record_stream(torch.cuda.current_stream())
+        else:
+            self.Qt = q64
         # the image may have been built on another stream: its blocks go back
         # to the allocator only after this stream's GEMM has read them
         for t in (self._Ct, self._gram):
             t.record_stream(torch.cuda.current_stream())
         self._Ct = self._gram = None
+        del q64
         self.Kp = K
         self.s = torch.empty(max(K, 1), dtype=F64, device=dev)
         self.u = torch.empty(max(K, 1), dtype=F64, device=dev)
-        self.work = torch.empty(int(_lib.load().mbem_multidot_work(max(K, 1), n)), dtype=F64,
-                                device=dev)
-        self.bytes = (K * n + K * K) * 8 + self.Wp.numel() * 8 + 2 * n * 8 + 8 * self.m * n
+        lib = _lib.load()
+        work = (lib.mbem_coarse_project_work(max(K, 1), n) if self.q32 else
+                lib.mbem_multidot_work(max(K, 1), n))
+        self.work = torch.empty(int(work), dtype=F64, device=dev)
+        self.bytes = (K * self.ldq * (4 if self.q32 else 8) + K * K * 8 + self.Wp.numel() * 8
+                      + 2 * n * 8 + 8 * self.m * n)
         return self
 
     def check(self):
@@ -228,9 +256,14 @@ class CoarseSpace:
         """s = Q^T r (kept in ``self.s``) and rest = r - Q s."""
         self.finish()
         Kp, n = self.Kp, self.n
-        rest.copy_(r)
         if Kp == 0:
+            rest.copy_(r)
             return rest
+        if self.q32:
+            call("mbem_coarse_project_f32", ptr(self.Qt), self.ldq, Kp, n, ptr(r), ptr(self.s),
+                 ptr(rest), ptr(self.work), stream())
+            return rest
+        rest.copy_(r)
         call("mbem_multidot", ptr(self.Qt), n, Kp, n, ptr(r), ptr(self.s), ptr(self.work), stream())
         call("mbem_multiaxpy", ptr(self.Qt), n, Kp, n, ptr(self.s), ptr(rest), -1.0, stream())
         return rest

@@ -100,7 +100,7 @@ class MixedOperator:
     """y = F x + N x on the device (see the module docstring)."""
 
     def __init__(self, assembled, storage: str = "bf16", k_near: float = 2.0, pattern=None,
-                 split: bool = False):
+                 split: bool = False, cast_chunks: int = 1):
         if storage not in ("bf16", "fp32", "int8"):
             raise ValueError(f"storage {storage!r} (expected 'bf16', 'fp32' or 'int8')")
         if split and storage != "int8":
@@ -136,13 +136,27 @@ class MixedOperator:
             self.E = torch.empty((n, self.lde), dtype=torch.uint8, device=dev) if split else None
             hold = (self.nbr, ptr(self.rowptr), ptr(self.col), self.nnzb, ptr(self.vals))
             call("mbem_near_hold", ptr(A), A.stride(0), *hold, 0, stream())
+            # the cast in row chunks, an event after each (``cast_events``):
+            # a consumer of the codes (the coarse image) can follow chunk by
+            # chunk on another stream while the cast goes on
+            self.cast_events = []
+            bounds = np.linspace(0, n, max(1, int(cast_chunks)) + 1).astype(np.int64)
             try:
-                if split:
-                    call("mbem_cast_i8e", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
-                         ptr(self.S), self.nseg, ptr(self.E), self.lde, stream())
-                else:
-                    call("mbem_cast_i8", ptr(A), A.stride(0), n, n, ptr(self.F), self.ldf,
-                         ptr(self.S), self.nseg, stream())
+                for r0, r1 in zip(bounds[:-1], bounds[1:]):
+                    r0, r1 = int(r0), int(r1)
+                    if r1 <= r0:
+                        continue
+                    if split:
+                        call("mbem_cast_i8e", ptr(A[r0:r1]), A.stride(0), r1 - r0, n,
+                             ptr(self.F[r0:r1]), self.ldf, ptr(self.S[r0:r1]), self.nseg,
+                             ptr(self.E[r0:r1]), self.lde, stream())
+                    else:
+                        call("mbem_cast_i8", ptr(A[r0:r1]), A.stride(0), r1 - r0, n,
+                             ptr(self.F[r0:r1]), self.ldf, ptr(self.S[r0:r1]), self.nseg,
+                             stream())
+                    ev = torch.cuda.Event()
+                    ev.record()
+                    self.cast_events.append((r0, r1, ev))
             finally:
                 # the near blocks go back into the caller's A whatever the
                 # cast returned: A is never left with them zeroed
@@ -158,6 +172,7 @@ class MixedOperator:
                  stream())
             self.far_bytes = n * n * (2 if self.bf16 else 4)
             self.E = None
+            self.cast_events = []
         self._lower = {}
         self.xf = torch.empty(max(n, 4), dtype=torch.float32, device=dev)   # FP32 copy of x
         # host time only: the casts are queued, not waited for (stream order
@@ -414,17 +429,21 @@ class MixedReport:
 
 
 _SIDE = {}
+# row chunks of the int8 cast when a coarse space follows it (its image of
+# each chunk runs beside the cast of the next)
+CAST_CHUNKS = 6
 
 
-def _side_stream():
-    """The side stream of the operator build, one per device for the life of
-    the process (default priority).  A new stream per solve would leave the
-    caching allocator's blocks of the last build (held per stream) unusable
-    by the next, so every solve would map its far-field copy afresh."""
-    dev = torch.cuda.current_device()
-    if dev not in _SIDE:
-        _SIDE[dev] = torch.cuda.Stream(device=dev)
-    return _SIDE[dev]
+def _side_stream(k: int = 0):
+    """Side stream ``k`` of the operator build, one per device for the life
+    of the process (default priority): 0 for the operator, 1 for the coarse
+    space.  A new stream per solve would leave the caching allocator's blocks
+    of the last build (held per stream) unusable by the next, so every solve
+    would map its far-field copy afresh."""
+    key = (torch.cuda.current_device(), k)
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=key[0])
+    return _SIDE[key]
 
 
 def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: float = 2.0,
@@ -446,14 +465,16 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     if outer == "split" and storage != "int8":
         raise ValueError("outer='split' needs storage='int8'")
     from .preconditioner import BlockGaussSeidel
-    # Two streams (both at the default priority).  The near pattern needs
+    # Three streams, all at the default priority.  The near pattern needs
     # the geometry only: its counting pass is queued on a side stream beside
-    # the assembly and read back after the block-Jacobi host work.  The block-Jacobi build's host work (cluster tree, launches)
-    # runs while the assembly occupies the device.  Once its gathers have
-    # read A (``gathered``), the operator build goes on the side stream (near
-    # blocks held out of A, int8 cast, blocks restored), queued before the
-    # inversions' long launch sequence, so its HBM-bound cast runs beside the
-    # inversions, which work on their own copies of the chunks.
+    # the assembly and read back after the block-Jacobi host work, which
+    # runs while the assembly occupies the device.  Once the block-Jacobi
+    # gathers have read A (``gathered``) and the chunk inversions are queued
+    # on the main stream, the operator build goes on the side stream (near
+    # blocks held out of A, the int8 cast in row chunks, blocks restored),
+    # and the coarse space on a second side stream, its image following the
+    # cast chunk by chunk; the inversions work on their own copies of the
+    # chunks.
     t0 = time.perf_counter()
     main = torch.cuda.current_stream()
     side = _side_stream()
@@ -463,20 +484,26 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
             near = NearPattern(assembled, k_near)     # counting pass queued, not waited for
     built = {}
 
+    i8 = storage == "int8"
+    side2 = _side_stream(1) if coarse else None
+
     def start_operator(gathered, chunk_dofs):
         pat = near.result() if near is not None else pattern
         side.wait_event(gathered)
         with torch.cuda.stream(side):
             built["op"] = MixedOperator(assembled, storage=storage, k_near=k_near, pattern=pat,
-                                        split=outer == "split")
-            if coarse:
-                # the coarse space behind the operator build on the side
-                # stream, queued before the inversions' long launch sequence,
-                # so its image pass and factorisation run beside the
-                # inversions: the image of the mixed operator itself (int8
-                # codes and near blocks; for BF16 / FP32 storage the FP64 A,
-                # whose near blocks the build has put back by then)
-                from .coarse import CoarseSpace
+                                        split=outer == "split",
+                                        cast_chunks=CAST_CHUNKS if (coarse and i8) else 1)
+        if coarse:
+            # the coarse space on a second side stream: the image of the
+            # mixed operator itself (int8 codes and near blocks) follows the
+            # cast chunk by chunk, so most of it runs under the HBM-bound
+            # cast; for BF16 / FP32 storage it is the FP64 A's image, which
+            # waits for the build to put A's near blocks back
+            from .coarse import CoarseSpace
+            if not i8:
+                side2.wait_stream(side)
+            with torch.cuda.stream(side2):
                 built["cs"] = CoarseSpace(assembled, chunk_dofs, basis=coarse,
                                           op=built["op"]).finish()
 
@@ -487,6 +514,8 @@ def solve_mixed(assembled, rtol: float = 1e-10, storage: str = "bf16", k_near: f
     cs = built.get("cs")
     t1 = time.perf_counter()
     main.wait_stream(side)
+    if side2 is not None:
+        main.wait_stream(side2)
     M.coupling = op                      # the lower couplings read the mixed operator
     M.finish()                           # one wait for the assembly and both builds
     P = M
