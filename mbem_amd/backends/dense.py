@@ -356,8 +356,6 @@ class AssembledDense:
         _lib.require_gpu()
         t0 = time.perf_counter()
         require_anchor_or_deflate(system, jump, deflate)
-        warn_half_jump_eps(system, eps, jump)
-        warn_collocation_near_fault(system, eps)
         self.timings = _Timings(checks_s=time.perf_counter() - t0)
         self.system = system
         self.layout = system.layout
@@ -388,6 +386,12 @@ class AssembledDense:
         self.timings["geometry_s"] = time.perf_counter() - t0
         self.timings["assembly_s"] = None
         self._build()
+        # the two checks that only warn run while the device assembles (the
+        # build returns once its kernels are queued)
+        t0 = time.perf_counter()
+        warn_half_jump_eps(system, eps, jump)
+        warn_collocation_near_fault(system, eps)
+        self.timings["checks_s"] += time.perf_counter() - t0
         # the resolver holds the report, not ``self``: no reference cycle keeps
         # a dropped assembly (and its n x n matrix) alive until a full GC
         rep = self.assembly
@@ -412,12 +416,13 @@ class AssembledDense:
                     eps_arr = kb.resolve_patch_eps(self.eps, p)
                     tv, nrm = kb._source_arrays(p.mesh)
                     self._eps_arr[id(p)] = eps_arr
-                    self._src[id(p)] = dv.Sources(torch.from_numpy(tv).to(dev),
-                                                  torch.from_numpy(nrm).to(dev),
-                                                  torch.from_numpy(eps_arr).to(dev))
+                    # pinned, non-blocking uploads (the staging ring)
+                    self._src[id(p)] = dv.Sources(_lib.upload(tv, torch.float64),
+                                                  _lib.upload(nrm, torch.float64),
+                                                  _lib.upload(eps_arr, torch.float64))
                     x = np.ascontiguousarray(p.collocation_points())
                     self._pts_host[id(p)] = x
-                    self._pts[id(p)] = torch.from_numpy(x).to(dev)
+                    self._pts[id(p)] = _lib.upload(x, torch.float64)
         self._patches = patches
 
     def collocation_points_of(self, patch) -> np.ndarray:
