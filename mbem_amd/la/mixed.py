@@ -359,7 +359,7 @@ class _RecycledInner:
         Hc = self.hc[:j_done, :k].cpu().numpy().T if k > 0 else None     # k x j_done
         # x += Z y - U (Hc y): one upload of both coefficient vectors
         coefs = np.concatenate([y, Hc @ y]) if k > 0 else y
-        cd = torch.as_tensor(coefs, dtype=F64).to(x.device, non_blocking=False)
+        cd = _lib.upload(coefs, F64)                        # pinned, no host wait
         ws.axpy_rows(Z, j_done, cd[:j_done], x, 1.0)
         if k > 0:
             ws.axpy_rows(self.U, k, cd[j_done:], x, -1.0)
@@ -379,21 +379,24 @@ class _RecycledInner:
         V, Z = ws.V, ws.Z
         # one upload of every coefficient column: rows of Q^T, Rinv^T, Hc^T
         parts = [Q[:, :add].T, Rinv[:, :add].T] + ([Hc.T] if Hc is not None and k > 0 else [])
-        cf = torch.as_tensor(np.ascontiguousarray(np.concatenate([p.reshape(-1) for p in parts])),
-                             dtype=F64).to(dev)
+        cf = _lib.upload(np.concatenate([p.reshape(-1) for p in parts]), F64)
         q_t = cf[:add * (j + 1)].view(add, j + 1)
         r_t = cf[add * (j + 1):add * (2 * j + 1)].view(add, j)
-        W = torch.empty((j, self.n), dtype=F64, device=dev)
-        W.copy_(Z[:j])
+        n = self.n
+        # three row-major FP64 products instead of a launch per vector (the
+        # per-vector form was ~5 j host-bound launches per solve)
         if Hc is not None and k > 0:
             h_t = cf[add * (2 * j + 1):].view(j, k)
-            for i in range(j):
-                ws.axpy_rows(self.U, k, h_t[i], W[i], -1.0)
-        for i in range(add):
-            self.C[k + i].zero_()
-            ws.axpy_rows(V, j + 1, q_t[i], self.C[k + i], 1.0)
-            self.U[k + i].zero_()
-            ws.axpy_rows(W, j, r_t[i], self.U[k + i], 1.0)
+            W = torch.empty((j, n), dtype=F64, device=dev)
+            W.copy_(Z[:j])
+            call("mbem_dgemm_rowmajor", j, n, k, -1.0, ptr(h_t), k, ptr(self.U),    # W = Z - Hc^T U
+                 self.U.stride(0), 1.0, ptr(W), W.stride(0), stream())
+        else:
+            W = Z[:j]
+        call("mbem_dgemm_rowmajor", add, n, j + 1, 1.0, ptr(q_t), j + 1, ptr(V), V.stride(0),
+             0.0, ptr(self.C[k]), self.C.stride(0), stream())           # C_new = Q^T V
+        call("mbem_dgemm_rowmajor", add, n, j, 1.0, ptr(r_t), j, ptr(W), W.stride(0),
+             0.0, ptr(self.U[k]), self.U.stride(0), stream())           # U_new = R^-T W
         self.k = k + add
 
 
