@@ -22,10 +22,11 @@
 //     outside the panel, W[i, j] = G[i, :] S[:, j] for the pivot rows (S the
 //     saved pivot rows): a rank-32 update, FP32 FMAs.
 //
-// Then k_gj_finish writes the inverse: logical row i, column j of the result
-// is W[map[i], sigma(j)], sigma the pivots' column swaps applied in reverse
-// order (the in-place Gauss-Jordan's row interchanges undone on the
-// columns), into the packed row-major inverse (pitch ldo, pads zero).
+// Then k_gj_sigma composes sigma, the pivots' column swaps applied in
+// reverse order (the in-place Gauss-Jordan's row interchanges undone on the
+// columns), and k_gj_finish writes the inverse: logical row i, column j of
+// the result is W[map[i], sigma(j)], into the packed row-major inverse (pitch
+// ldo, pads zero), a workgroup per 16 rows of a chunk.
 //
 // A zero pivot records info = its column + 1 (the first) and continues with
 // r = 0, so nothing becomes NaN; the caller reads info and raises.
@@ -312,38 +313,59 @@ __global__ __launch_bounds__(256) void k_gj_update(const int64_t* __restrict__ d
     }
 }
 
-// the inverse into the packed output: out[i, j] = W[map[i], sigma(j)] (pads
-// zero), sigma from the pivots in reverse order.  One workgroup per chunk.
-__global__ __launch_bounds__(kNT) void k_gj_finish(const int64_t* __restrict__ desc,
-                                                   const float* __restrict__ W,
-                                                   const int* __restrict__ rowmap,
-                                                   const int* __restrict__ piv, float* __restrict__ out)
+// sigma, the pivots' column swaps composed in reverse order, written over
+// the chunk's pivots (working memory).  One wave per chunk; the serial part
+// runs on LDS.
+__global__ __launch_bounds__(64) void k_gj_sigma(const int64_t* __restrict__ desc,
+                                                 int* __restrict__ piv)
 {
     const int64_t* d = desc + (long)blockIdx.x * D_N;
     const long m = d[D_M];
     __shared__ int sig[kMaxM];
     __shared__ int pv[kMaxM];
-    const int tid = threadIdx.x;
-    for (long j = tid; j < m; j += kNT) {
+    int* p = piv + d[D_MAP];
+    const int lane = threadIdx.x;
+    for (long j = lane; j < m; j += 64) {
         sig[j] = (int)j;
-        pv[j] = piv[d[D_MAP] + j];
+        pv[j] = p[j];
     }
     __syncthreads();
-    if (tid == 0) {
+    if (lane == 0) {
         for (long t = m - 1; t >= 0; --t) {
-            const int p = pv[t];
+            const int q = pv[t];
             const int a = sig[t];
-            sig[t] = sig[p];
-            sig[p] = a;
+            sig[t] = sig[q];
+            sig[q] = a;
         }
     }
+    __syncthreads();
+    for (long j = lane; j < m; j += 64) p[j] = sig[j];
+}
+
+// out_c[i, j] = W_c[map[i], sigma(j)] for rows i0 .. i0 + kFinRows of chunk
+// blockIdx.y, sigma read from piv (k_gj_sigma) into LDS once per block
+constexpr int kFinRows = 16;
+__global__ __launch_bounds__(256) void k_gj_finish(const int64_t* __restrict__ desc,
+                                                   const float* __restrict__ W,
+                                                   const int* __restrict__ rowmap,
+                                                   const int* __restrict__ sigma,
+                                                   float* __restrict__ out)
+{
+    const int64_t* d = desc + (long)blockIdx.y * D_N;
+    const long m = d[D_M];
+    const long i0 = (long)blockIdx.x * kFinRows;
+    if (i0 >= m) return;                       // uniform across the block
+    __shared__ int sig[kMaxM];
+    const int tid = threadIdx.x;
+    for (long j = tid; j < m; j += 256) sig[j] = sigma[d[D_MAP] + j];
     __syncthreads();
     const long ldw = d[D_LDW], ldo = d[D_LDO];
     const float* Wc = W + d[D_WOFF];
     const int* map = rowmap + d[D_MAP];
     float* o = out + d[D_OOFF];
     const int lane = tid & 63, wave = tid >> 6;
-    for (long i = wave; i < m; i += kNT / 64) {
+    const long i1 = (i0 + kFinRows < m) ? i0 + kFinRows : m;
+    for (long i = i0 + wave; i < i1; i += 4) {
         const float* w = Wc + (long)map[i] * ldw;
         for (long j = lane; j < ldo; j += 64) o[i * ldo + j] = (j < m) ? w[sig[j]] : 0.0f;
     }
@@ -417,8 +439,12 @@ int mbem_bj_invert_f32(const int64_t* desc_host, const int64_t* desc, int64_t nc
         hipLaunchKernelGGL(k_gj_update, ugrid, dim3(256), 0, stream, desc, k, tiles, W,
                            (const int*)rowmap, (const float*)saved);
     }
-    hipLaunchKernelGGL(k_gj_finish, pgrid, dim3(kNT), 0, stream, desc, (const float*)W,
-                       (const int*)rowmap, (const int*)piv, out);
+    // sigma composed once per chunk (into piv), then the rows copied by
+    // every CU: one workgroup per 16 rows of a chunk
+    hipLaunchKernelGGL(k_gj_sigma, pgrid, dim3(64), 0, stream, desc, piv);
+    hipLaunchKernelGGL(k_gj_finish, dim3((unsigned)((mmax + kFinRows - 1) / kFinRows), (unsigned)nchunks),
+                       dim3(256), 0, stream, desc, (const float*)W, (const int*)rowmap,
+                       (const int*)piv, out);
     return launch_status();
 }
 
