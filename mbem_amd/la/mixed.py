@@ -325,7 +325,8 @@ class _RecycledInner:
             axpby(1.0 / beta, rr, 0.0, V[0])
             g[0] = beta
             for j in range(m):
-                self.M.apply_into(V[j], Z[j])
+                if j == 0:
+                    self.M.apply_into(V[j], Z[j])
                 w = V[j + 1]
                 self.op.apply_into(Z[j], w)
                 if k > 0:
@@ -333,7 +334,12 @@ class _RecycledInner:
                     call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(w), ptr(hc),
                          ptr(self.cwork), stream())
                     ws.axpy_rows(self.C, k, hc, w, -1.0)
-                hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True)
+                # the next step's preconditioner apply is queued before the
+                # host waits for this step's coefficients (it only needs the
+                # normalized V[j + 1], already queued); unused if the solve
+                # stops here.  Z has m rows: the last step queues none.
+                nxt = (lambda j=j: self.M.apply_into(V[j + 1], Z[j + 1])) if j + 1 < m else None
+                hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True, before_wait=nxt)
                 H[:j + 1, j] = hsum
                 H[j + 1, j] = wnorm
                 Hraw[:j + 2, j] = H[:j + 2, j]

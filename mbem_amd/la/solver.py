@@ -167,6 +167,8 @@ class _Krylov:
         self.h = torch.empty(self.h_len + tail, dtype=F64, device=dev)
         self.work = torch.empty(int(_lib.load().mbem_multidot_work(m + 1, n)), dtype=F64,
                                 device=dev)
+        self._h_host = None          # pinned copy of h (cgs2 with before_wait)
+        self._h_ready = None
 
     def dots(self, rows: torch.Tensor, k: int, w: torch.Tensor, at: int = 0) -> torch.Tensor:
         """h[at:at+k] = rows[:k] . w on the device (no host sync)."""
@@ -181,13 +183,16 @@ class _Krylov:
         return self.h[self.h_len:self.h_len + e]
 
     def cgs2(self, V: torch.Tensor, k: int, w: torch.Tensor, normalize: bool = False,
-             tail: int = 0):
+             tail: int = 0, before_wait=None):
         """Project w against V[:k] twice (classical Gram-Schmidt, reorthogonalized)
         and return (h, ||w||) on the host after ONE synchronization; with
         ``normalize`` w is also divided by its norm on the device (when the
         norm exceeds 1e-300) before that synchronization.  With ``tail`` = e
         the first e values of ``tail_view`` come back in the same transfer,
-        as a third element."""
+        as a third element.  ``before_wait`` (a callable) is called once the
+        transfer is queued and before the host waits for it: work it queues
+        runs on the device while the host waits (the transfer waits only for
+        the work queued before it)."""
         h1 = self.dots(V, k, w, 0)
         self.axpy_rows(V, k, h1, w, -1.0)
         h2 = self.dots(V, k, w, k)
@@ -195,10 +200,20 @@ class _Krylov:
         self.dots(w.view(1, -1), 1, w, 2 * k)
         if normalize:
             call("mbem_normalize", self.n, ptr(self.h[2 * k:]), ptr(w), stream())
+        e = self.h_len + tail if tail else 2 * k + 1
+        if before_wait is None:
+            hv = self.h[:e].cpu().numpy()
+        else:
+            if self._h_host is None:
+                self._h_host = torch.empty(self.h.numel(), dtype=F64, pin_memory=True)
+                self._h_ready = torch.cuda.Event()
+            self._h_host[:e].copy_(self.h[:e], non_blocking=True)
+            self._h_ready.record()
+            before_wait()
+            self._h_ready.synchronize()
+            hv = self._h_host[:e].numpy().copy()
         if not tail:
-            hv = self.h[:2 * k + 1].cpu().numpy()
             return hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0)))
-        hv = self.h[:self.h_len + tail].cpu().numpy()
         return (hv[:k] + hv[k:2 * k], float(np.sqrt(max(hv[2 * k], 0.0))),
                 hv[self.h_len:self.h_len + tail].copy())
 
