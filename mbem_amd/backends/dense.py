@@ -89,6 +89,16 @@ class DenseBackend:
                               lu=self.lu, calibrate_all_rows=self.calibrate_all_rows)
 
 
+def _centroid_collocated(p) -> bool:
+    """Whether the patch collocates at its triangles' centroids through the
+    stock Patch and TriMesh methods (kb._source_geometry then gives its source
+    arrays and collocation points in one pass, bit for bit)."""
+    from ..mesh import TriMesh
+    from ..model.core import Patch
+    return (type(p).collocation_points is Patch.collocation_points
+            and type(p).nodes is Patch.nodes and type(p.mesh) is TriMesh)
+
+
 def require_anchor_or_deflate(system: BlockSystem, jump: str, deflate: bool):
     """dense.py:101-121."""
     if jump == "calibrated" and not deflate and not system.model.is_anchored() \
@@ -414,13 +424,16 @@ class AssembledDense:
                 if id(p) not in self._src:
                     patches.append(p)
                     eps_arr = kb.resolve_patch_eps(self.eps, p)
-                    tv, nrm = kb._source_arrays(p.mesh)
+                    if _centroid_collocated(p):
+                        tv, nrm, x = kb._source_geometry(p.mesh)
+                    else:
+                        tv, nrm = kb._source_arrays(p.mesh)
+                        x = np.ascontiguousarray(p.collocation_points())
                     self._eps_arr[id(p)] = eps_arr
                     # pinned, non-blocking uploads (the staging ring)
                     self._src[id(p)] = dv.Sources(_lib.upload(tv, torch.float64),
                                                   _lib.upload(nrm, torch.float64),
                                                   _lib.upload(eps_arr, torch.float64))
-                    x = np.ascontiguousarray(p.collocation_points())
                     self._pts_host[id(p)] = x
                     self._pts[id(p)] = _lib.upload(x, torch.float64)
         self._patches = patches

@@ -74,3 +74,21 @@ def test_mesh_arrays_memo():
         kb.assemble_traction_matrix(pts, pts, m, mat, 0.1, kind="bogus")
     with pytest.raises(ValueError, match="ids shape"):
         kb.assemble_traction_matrix(pts, pts, m, mat, 0.1, order=1, ids=np.zeros((1, 1)), n_cols=1)
+
+
+def test_source_geometry_matches_the_separate_calls_bit_for_bit():
+    """kb._source_geometry (one gather) against _source_arrays plus the
+    patch's collocation points, on every patch of a box with a dipping
+    fault: equal arrays, not merely close."""
+    from mbem_amd.backends.dense import _centroid_collocated
+    from mbem_amd.cases import box_fault_model
+    from mbem_amd.kernels import basis as kb
+    model, _, _ = box_fault_model(8, fault_cells=(6, 10))
+    patches = [p for r in model.regions for p in list(r.patches) + list(r.faults)]
+    assert patches and all(_centroid_collocated(p) for p in patches)
+    for p in patches:
+        tv, nrm, x = kb._source_geometry(p.mesh)
+        tv0, nrm0 = kb._source_arrays(p.mesh)
+        assert np.array_equal(tv, tv0) and np.array_equal(nrm, nrm0)
+        assert np.array_equal(x, np.ascontiguousarray(p.collocation_points()))
+        assert tv.flags.c_contiguous and nrm.flags.c_contiguous and x.flags.c_contiguous
