@@ -59,6 +59,17 @@ extern "C" {
 /* doubles per source record */
 int mbem_src_fields(void);
 
+/* Host function (no device work): from a triangle mesh's vertices (nv,3)
+ * and triangles (nt,3, int64 vertex indices) write tri_verts (nt,3,3), the
+ * unit normals (nt,3) and the centroids (nt,3), each row-major: the source
+ * arrays and centroid collocation points of a P0 patch, bit for bit what
+ * numpy gives for vertices[triangles], cross / norm and mean (plain IEEE
+ * FP64 operations, no contraction).  Replaces the host geometry of
+ * basis.py:91-150 (_source_arrays and the centroids).  MBEM_EINVAL on a
+ * triangle index outside [0, nv). */
+int mbem_tri_geometry(const double *verts, int64_t nv, const int64_t *tris, int64_t nt,
+                      double *tri_verts, double *normals, double *centroids);
+
 /* Pack sources: tri_verts (ns,3,3), normals (ns,3), eps (ns) -> src
  * (mbem_src_fields(), ld_src).  Replaces tri_kernels._precompute_frames
  * (tri_kernels.py:1052-1065) and the per-source _tri_frame (:136-153). */

@@ -72,6 +72,7 @@ _SIGS = {
     "mbem_vec_to_f32": [_p, _i64, _p, _p],
     "mbem_gemv_lo": [_p, _i64, _i64, _i64, _p, _p, _d, _d, _int, _p],
     "mbem_i8_segments": [_i64],
+    "mbem_tri_geometry": [_p, _i64, _p, _i64, _p, _p, _p],
     "mbem_cast_i8": [_p, _i64, _i64, _i64, _p, _i64, _p, _i64, _p],
     "mbem_near_hold": [_p, _i64, _i64, _p, _p, _i64, _p, _int, _p],
     "mbem_gemv_i8": [_p, _i64, _p, _i64, _i64, _i64, _p, _p, _d, _d, _p],
@@ -296,6 +297,13 @@ def ptr(t) -> ctypes.c_void_p:
     if t is None:
         return ctypes.c_void_p(0)
     return ctypes.c_void_p(t.data_ptr())
+
+
+def host_ptr(a: np.ndarray) -> ctypes.c_void_p:
+    """Address of a C-contiguous host array (for the library's host functions)."""
+    if not a.flags.c_contiguous:
+        raise ValueError("host arrays passed to the library must be C-contiguous")
+    return ctypes.c_void_p(a.ctypes.data)
 
 
 def check(status: int, what: str) -> None:

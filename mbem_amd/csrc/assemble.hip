@@ -13,6 +13,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
+
 #include "pair.hpp"
 #include "../../include/mbem_hip.h"
 
@@ -582,6 +584,41 @@ int mbem_add_block_diag(double* A, int64_t lda, int64_t nc, const double* D, dou
     if (nc == 0) return MBEM_OK;
     hipLaunchKernelGGL(k_add_block_diag, dim3((unsigned)((9 * nc + 255) / 256)), dim3(256), 0, stream, A, lda, nc, D, scale);
     return launch_status();
+}
+
+// the same operations and order as numpy's vertices[triangles], np.cross,
+// sqrt(add.reduce(c * c)) and mean over the three vertices (host FP64, no
+// contraction into FMAs)
+int mbem_tri_geometry(const double* verts, int64_t nv, const int64_t* tris, int64_t nt,
+                      double* tri_verts, double* normals, double* centroids)
+{
+#pragma clang fp contract(off)
+    if (nv < 0 || nt < 0 || (nt > 0 && (!verts || !tris || !tri_verts || !normals || !centroids)))
+        return MBEM_EINVAL;
+    for (int64_t t = 0; t < 3 * nt; ++t)
+        if (tris[t] < 0 || tris[t] >= nv) return MBEM_EINVAL;
+    for (int64_t t = 0; t < nt; ++t) {
+        const double* v0 = verts + 3 * tris[3 * t];
+        const double* v1 = verts + 3 * tris[3 * t + 1];
+        const double* v2 = verts + 3 * tris[3 * t + 2];
+        double* tv = tri_verts + 9 * t;
+        for (int k = 0; k < 3; ++k) {
+            tv[k] = v0[k];
+            tv[3 + k] = v1[k];
+            tv[6 + k] = v2[k];
+        }
+        const double a0 = v1[0] - v0[0], a1 = v1[1] - v0[1], a2 = v1[2] - v0[2];
+        const double b0 = v2[0] - v0[0], b1 = v2[1] - v0[1], b2 = v2[2] - v0[2];
+        const double c0 = a1 * b2 - a2 * b1;
+        const double c1 = a2 * b0 - a0 * b2;
+        const double c2 = a0 * b1 - a1 * b0;
+        const double len = std::sqrt((c0 * c0 + c1 * c1) + c2 * c2);
+        normals[3 * t] = c0 / len;
+        normals[3 * t + 1] = c1 / len;
+        normals[3 * t + 2] = c2 / len;
+        for (int k = 0; k < 3; ++k) centroids[3 * t + k] = ((v0[k] + v1[k]) + v2[k]) / 3.0;
+    }
+    return MBEM_OK;
 }
 
 }  // extern "C"

@@ -100,19 +100,22 @@ def _source_arrays(mesh_source):
 
 
 def _source_geometry(mesh_source):
-    """(tri_verts, normals, centroids) of a triangle mesh from one gather of
-    its vertices: bit for bit what _source_arrays and mesh.centroids() give
-    (the same operations in the same order, np.cross written out), at one
-    gather instead of three."""
-    tv = np.asarray(mesh_source.vertices, float)[np.asarray(mesh_source.triangles)]
-    a = tv[:, 1] - tv[:, 0]
-    b = tv[:, 2] - tv[:, 0]
-    cr = np.empty_like(a)
-    cr[:, 0] = a[:, 1] * b[:, 2] - a[:, 2] * b[:, 1]
-    cr[:, 1] = a[:, 2] * b[:, 0] - a[:, 0] * b[:, 2]
-    cr[:, 2] = a[:, 0] * b[:, 1] - a[:, 1] * b[:, 0]
-    nrm = cr / np.sqrt(np.add.reduce(cr * cr, axis=1))[:, None]
-    return np.ascontiguousarray(tv), np.ascontiguousarray(nrm), np.ascontiguousarray(tv.mean(axis=1))
+    """(tri_verts, normals, centroids) of a triangle mesh in one host loop of
+    the library (mbem_tri_geometry): bit for bit what _source_arrays and
+    mesh.centroids() give (the same FP64 operations in the same order)."""
+    from .. import _lib
+    verts = np.ascontiguousarray(mesh_source.vertices, dtype=np.float64)
+    tris = np.ascontiguousarray(mesh_source.triangles, dtype=np.int64)
+    if verts.ndim != 2 or verts.shape[1] != 3 or tris.ndim != 2 or tris.shape[1] != 3:
+        raise ValueError(f"mesh arrays must be (nv, 3) and (nt, 3), got {verts.shape} and "
+                         f"{tris.shape}")
+    nt = tris.shape[0]
+    tv = np.empty((nt, 3, 3))
+    nrm = np.empty((nt, 3))
+    cen = np.empty((nt, 3))
+    _lib.call("mbem_tri_geometry", _lib.host_ptr(verts), verts.shape[0], _lib.host_ptr(tris), nt,
+              _lib.host_ptr(tv), _lib.host_ptr(nrm), _lib.host_ptr(cen))
+    return tv, nrm, cen
 
 
 def _field_points(mesh_or_points):

@@ -92,3 +92,27 @@ def test_source_geometry_matches_the_separate_calls_bit_for_bit():
         assert np.array_equal(tv, tv0) and np.array_equal(nrm, nrm0)
         assert np.array_equal(x, np.ascontiguousarray(p.collocation_points()))
         assert tv.flags.c_contiguous and nrm.flags.c_contiguous and x.flags.c_contiguous
+
+
+def test_tri_geometry_on_irregular_triangles_and_bad_indices():
+    """mbem_tri_geometry (host code of the library) against numpy on random
+    triangles spanning ten decades of size: equal arrays; an index outside
+    the vertex list is refused before anything is written."""
+    from mbem_amd import _lib
+    from mbem_amd.kernels import basis as kb
+    from mbem_amd.mesh import TriMesh
+    rng = np.random.default_rng(7)
+    verts = rng.standard_normal((300, 3)) * np.exp(rng.uniform(-12, 12, (300, 1)))
+    tris = np.array([rng.choice(300, 3, replace=False) for _ in range(500)])
+    mesh = TriMesh(verts, tris)
+    tv, nrm, cen = kb._source_geometry(mesh)
+    tv0, nrm0 = kb._source_arrays(mesh)
+    assert np.array_equal(tv, tv0) and np.array_equal(nrm, nrm0)
+    assert np.array_equal(cen, mesh.centroids())
+    bad = np.ascontiguousarray(tris, dtype=np.int64)
+    bad[3, 1] = 300
+    out = np.full((500, 3), 7.0)
+    st = _lib.load().mbem_tri_geometry(_lib.host_ptr(np.ascontiguousarray(verts)), 300,
+                                        _lib.host_ptr(bad), 500, _lib.host_ptr(np.empty((500, 9))),
+                                        _lib.host_ptr(out), _lib.host_ptr(np.empty((500, 3))))
+    assert st == _lib.MBEM_EINVAL and np.all(out == 7.0)
