@@ -305,17 +305,22 @@ class _RecycledInner:
         x = z
         x.zero_()
         rr = r.clone()
-        r_norm = ws.norm(rr)
-        if r_norm == 0.0:
-            self.iterations.append(0)
-            return z
+        # ||r|| and, after the projection onto span C, ||rr|| reach the host
+        # in one transfer (the projection of a zero r is zero)
+        ws.dots(rr.view(1, -1), 1, rr, 0)
         if k > 0:
             coef = torch.empty(k, dtype=F64, device=rr.device)
             call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(rr), ptr(coef),
                  ptr(self.cwork), stream())
             ws.axpy_rows(self.U, k, coef, x, 1.0)
             ws.axpy_rows(self.C, k, coef, rr, -1.0)
-        beta = ws.norm(rr)
+        ws.dots(rr.view(1, -1), 1, rr, 1)
+        hv = ws.h[:2].cpu().numpy()
+        r_norm = float(np.sqrt(max(float(hv[0]), 0.0)))
+        if r_norm == 0.0:
+            self.iterations.append(0)
+            return z
+        beta = float(np.sqrt(max(float(hv[1]), 0.0)))
         V, Z = ws.V, ws.Z
         H = np.zeros((m + 1, m))
         Hraw = np.zeros((m + 1, m))
