@@ -290,10 +290,9 @@ class _RecycledInner:
         self.U = torch.zeros((self.kmax, n), dtype=F64, device=dev)
         self.C = torch.zeros((self.kmax, n), dtype=F64, device=dev)
         self.k = 0
-        self.ws = _Krylov(n, maxiter, dev)
-        # row j: the C-projection coefficients of Arnoldi step j (kept on the
-        # device, read once per solve)
-        self.hc = torch.zeros((maxiter, max(self.kmax, 1)), dtype=F64, device=dev)
+        # the C-projection coefficients of an Arnoldi step sit in the tail of
+        # the CGS2 buffer and reach the host with the step's own coefficients
+        self.ws = _Krylov(n, maxiter, dev, tail=max(self.kmax, 1))
         self.cwork = torch.empty(int(_lib.load().mbem_multidot_work(max(self.kmax, 1), n)),
                                  dtype=F64, device=dev)
         self.iterations = []
@@ -325,6 +324,7 @@ class _RecycledInner:
         H = np.zeros((m + 1, m))
         Hraw = np.zeros((m + 1, m))
         cs, sn, g = np.zeros(m), np.zeros(m), np.zeros(m + 1)
+        Hc_rows = np.zeros((m, k))
         j_done = 0
         if beta / r_norm > self.rtol:
             axpby(1.0 / beta, rr, 0.0, V[0])
@@ -335,7 +335,7 @@ class _RecycledInner:
                 w = V[j + 1]
                 self.op.apply_into(Z[j], w)
                 if k > 0:
-                    hc = self.hc[j]
+                    hc = ws.tail_view(k)
                     call("mbem_multidot", ptr(self.C), self.C.stride(0), k, n, ptr(w), ptr(hc),
                          ptr(self.cwork), stream())
                     ws.axpy_rows(self.C, k, hc, w, -1.0)
@@ -344,7 +344,11 @@ class _RecycledInner:
                 # normalized V[j + 1], already queued); unused if the solve
                 # stops here.  Z has m rows: the last step queues none.
                 nxt = (lambda j=j: self.M.apply_into(V[j + 1], Z[j + 1])) if j + 1 < m else None
-                hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True, before_wait=nxt)
+                if k > 0:
+                    hsum, wnorm, Hc_rows[j] = ws.cgs2(V, j + 1, w, normalize=True, tail=k,
+                                                      before_wait=nxt)
+                else:
+                    hsum, wnorm = ws.cgs2(V, j + 1, w, normalize=True, before_wait=nxt)
                 H[:j + 1, j] = hsum
                 H[j + 1, j] = wnorm
                 Hraw[:j + 2, j] = H[:j + 2, j]
@@ -367,7 +371,7 @@ class _RecycledInner:
         if j_done == 0:
             return z
         y = _solve_upper(H[:j_done, :j_done], g[:j_done])
-        Hc = self.hc[:j_done, :k].cpu().numpy().T if k > 0 else None     # k x j_done
+        Hc = Hc_rows[:j_done].T if k > 0 else None                        # k x j_done
         # x += Z y - U (Hc y): one upload of both coefficient vectors
         coefs = np.concatenate([y, Hc @ y]) if k > 0 else y
         cd = _lib.upload(coefs, F64)                        # pinned, no host wait
